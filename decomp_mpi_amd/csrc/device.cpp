@@ -1,0 +1,87 @@
+// HIP runtime plumbing: device detection, allocation, copies, streams.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "internal.h"
+
+namespace dm {
+
+static void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "decomp_mpi_amd: %s failed: %s\n", what, hipGetErrorString(e));
+    fflush(stderr);
+    abort();
+  }
+}
+
+bool device_available() {
+  static int cached = -1;
+  if (cached < 0) {
+    int n = 0;
+    cached = (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
+  }
+  return cached == 1;
+}
+
+void check_device(dm_comm_t comm) {
+  if (!device_available()) fail(comm, "no HIP device is available");
+}
+
+void *device_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  return p;
+}
+
+void device_free(void *p) {
+  if (p != nullptr) hip_check(hipFree(p), "hipFree");
+}
+
+void h2d(void *dst, const void *src, size_t bytes) {
+  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+void d2h(void *dst, const void *src, size_t bytes) {
+  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+void *world_stream(dm_world *w) {
+  if (w->stream == nullptr) {
+    hipStream_t s;
+    // Default priority on purpose: every stream of this library keeps it.
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    w->stream = (void *)s;
+    w->own_stream = true;
+  }
+  return w->stream;
+}
+
+void stream_sync(void *stream) {
+  hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+void world_finish(dm_world *w) {
+  if (!w->async) stream_sync(world_stream(w));
+}
+
+void destroy_stream(void *stream) {
+  hipStreamSynchronize((hipStream_t)stream);
+  hipStreamDestroy((hipStream_t)stream);
+}
+
+}  // namespace dm
+
+extern "C" int dm_device_available(void) { return dm::device_available() ? 1 : 0; }
+
+extern "C" int dm_fill(void *dst, int value, size_t bytes_per_rank) {
+  dm::Buffer b;
+  size_t off;
+  if (!dm::resolve(dst, &b, &off) || off + bytes_per_rank > b.slot) return DM_ERR_ARG;
+  for (int r = 0; r < b.nranks; r++)
+    dm::hip_check(hipMemset(b.base + (size_t)r * b.slot + off, value, bytes_per_rank), "hipMemset");
+  dm::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return DM_SUCCESS;
+}

@@ -1,0 +1,242 @@
+// Pass programs: tile axes, walk modes, radix schedules and device tables.
+#include "fft_program.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace dm {
+
+Side make_side(const HostLayout &h, i64 slot) {
+  Side s;
+  s.h = h;
+  s.slot = slot;
+  s.abs.resize(h.ndim);
+  s.order.resize(h.ndim);
+  for (int d = 0; d < h.ndim; d++) {
+    s.abs[d].resize(h.L[d]);
+    for (int g = 0; g < h.L[d]; g++) s.abs[d][g] = (i64)h.rank_part[d][g] * slot + h.m[d] * h.off[d][g];
+    s.order[d].resize(h.L[d]);
+    std::iota(s.order[d].begin(), s.order[d].end(), 0);
+    const auto &a = s.abs[d];
+    std::stable_sort(s.order[d].begin(), s.order[d].end(), [&](int x, int y) { return a[x] < a[y]; });
+  }
+  return s;
+}
+
+// Odd prime factors one stage each; powers of two in radix-8 stages with one
+// 16/4/2 stage for the remainder.
+bool choose_radices(int n, std::vector<int> *out) {
+  out->clear();
+  if (n < 1) return false;
+  std::vector<int> odd;
+  const int primes[] = {13, 11, 7, 5, 3};
+  for (int p : primes)
+    while (n % p == 0) {
+      odd.push_back(p);
+      n /= p;
+    }
+  int k = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    k++;
+  }
+  if (n != 1) return false;
+  std::vector<int> pw;
+  while (k >= 3) {
+    pw.push_back(8);
+    k -= 3;
+  }
+  if (k == 1) {
+    if (!pw.empty()) pw.back() = 16;
+    else pw.push_back(2);
+  } else if (k == 2) {
+    pw.push_back(4);
+  }
+  out->insert(out->end(), pw.begin(), pw.end());
+  out->insert(out->end(), odd.begin(), odd.end());
+  return true;
+}
+
+std::vector<double> roots(int n) {
+  std::vector<double> w(2 * (size_t)n);
+  const long double tau = 6.283185307179586476925286766559005768L;
+  const long double h = 0.70710678118654752440084436210484903L;
+  const long double tc[8] = {1, h, 0, -h, -1, -h, 0, h};
+  const long double ts[8] = {0, h, 1, h, 0, -h, -1, -h};
+  for (int k = 0; k < n; k++) {
+    long double c, s;
+    if ((8LL * k) % n == 0) {  // multiples of pi/4 exactly
+      c = tc[8LL * k / n];
+      s = ts[8LL * k / n];
+    } else {
+      long double a = tau * (long double)k / (long double)n;
+      c = cosl(a);
+      s = sinl(a);
+    }
+    w[2 * (size_t)k] = (double)c;
+    w[2 * (size_t)k + 1] = (double)(-s);
+  }
+  return w;
+}
+
+size_t TableBuilder::put(const void *src, size_t bytes) {
+  size_t at = host.size();
+  host.resize(at + ((bytes + 255) & ~(size_t)255), 0);
+  memcpy(&host[at], src, bytes);
+  return at;
+}
+
+static bool is_identity(const std::vector<int> &o) {
+  for (size_t i = 0; i < o.size(); i++)
+    if (o[i] != (int)i) return false;
+  return true;
+}
+
+bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, bool inverse, const void *in,
+               void *out, TableBuilder &tb, PendingPass *pp) {
+  int nd = I.h.ndim;
+  FftLaunch &L = pp->L;
+  memset(&L, 0, sizeof L);
+  L.ndim = nd;
+  for (int d = 0; d < nd; d++) L.L[d] = I.h.L[d];
+  L.t = t;
+  L.N = I.h.L[t];
+  int cap = fft_tile_capacity(dbl);
+  if (L.N > cap) {
+    raise_error(comm, "FFT: axis %d of length %d exceeds the %d-point tile of this build", t, L.N, cap);
+    return false;
+  }
+  std::vector<int> rad;
+  if (!choose_radices(L.N, &rad) || (int)rad.size() > 16) {
+    raise_error(comm, "FFT: axis length %d has a prime factor above 13", L.N);
+    return false;
+  }
+  L.nstage = (int)rad.size();
+  for (int s = 0; s < L.nstage; s++) L.radix[s] = rad[s];
+  auto fastest_other = [&](const Side &s) {
+    int best = -1;
+    for (int d = 0; d < nd; d++) {
+      if (d == t || s.h.L[d] <= 1) continue;
+      if (best < 0 || s.h.m[d] < s.h.m[best] || (s.h.m[d] == s.h.m[best] && d > best)) best = d;
+    }
+    return best;
+  };
+  bool in_row = (I.h.fastest == t) || fastest_other(I) < 0;
+  bool out_row = (U.h.fastest == t) || fastest_other(U) < 0;
+  int u = out_row ? fastest_other(I) : fastest_other(U);
+  L.u = u;
+  L.in_row = in_row ? 1 : 0;
+  L.out_row = out_row ? 1 : 0;
+  if (u >= 0) {
+    L.T = std::max(1, std::min(std::min(cap / L.N, I.h.L[u]), 512));
+    L.tiles_u = (I.h.L[u] + L.T - 1) / L.T;
+  } else {
+    L.T = 1;
+    L.tiles_u = 1;
+  }
+  L.nbatch = 0;
+  for (int d = 0; d < nd; d++)
+    if (d != t && d != u) L.batch_axis[L.nbatch++] = d;
+  long long batch = 1;
+  for (int k = 0; k < L.nbatch; k++) batch *= I.h.L[L.batch_axis[k]];
+  if ((long long)L.tiles_u * batch >= (1LL << 31)) {
+    raise_error(comm, "FFT: %lld workgroups exceed the launch limit", (long long)L.tiles_u * batch);
+    return false;
+  }
+  L.inverse = inverse ? 1 : 0;
+  L.dbl = dbl;
+  for (int d = 0; d < nd; d++) {
+    pp->in_off[d] = tb.put(I.abs[d].data(), I.abs[d].size() * sizeof(i64));
+    pp->out_off[d] = tb.put(U.abs[d].data(), U.abs[d].size() * sizeof(i64));
+  }
+  // rank-dependent strides: owner, local offset and per-rank stride tables
+  auto side = [&](const Side &s, FftSide &fs, PendingSide &ps) {
+    fs.uniform = s.h.uniform ? 1 : 0;
+    fs.proc_used = s.h.proc_used;
+    fs.slot = s.slot;
+    if (s.h.uniform) return;
+    for (int d = 0; d < nd; d++) {
+      ps.rank[d] = tb.put(s.h.rank_part[d].data(), s.h.L[d] * sizeof(int));
+      ps.off[d] = tb.put(s.h.off[d].data(), s.h.L[d] * sizeof(int));
+    }
+    ps.strides = tb.put(s.h.strides.data(), s.h.strides.size() * sizeof(i64));
+  };
+  side(I, L.sin, pp->sin);
+  side(U, L.sout, pp->sout);
+  if (in_row && !is_identity(I.order[t])) pp->ord_in = (long)tb.put(I.order[t].data(), L.N * sizeof(int));
+  if (out_row && !is_identity(U.order[t])) pp->ord_out = (long)tb.put(U.order[t].data(), L.N * sizeof(int));
+  if (u >= 0) {
+    // tile lines follow the address order of the column-walked side
+    const Side &cs = out_row ? I : U;
+    if (!is_identity(cs.order[u])) pp->ord_u = (long)tb.put(cs.order[u].data(), I.h.L[u] * sizeof(int));
+  }
+  std::vector<double> w = roots(L.N);
+  if (dbl) {
+    pp->tw = (long)tb.put(w.data(), w.size() * sizeof(double));
+  } else {
+    std::vector<float> wf(w.begin(), w.end());
+    pp->tw = (long)tb.put(wf.data(), wf.size() * sizeof(float));
+  }
+  L.in = in;
+  L.out = out;
+  return true;
+}
+
+void *finalize_passes(dm_comm_t comm, TableBuilder &tb, std::vector<PendingPass> &pp,
+                      std::vector<FftLaunch> *out) {
+  void *mem = device_alloc(std::max<size_t>(tb.host.size(), 256));
+  if (mem == nullptr) {
+    raise_error(comm, "FFT: device allocation of %zu bytes of tables failed", tb.host.size());
+    return nullptr;
+  }
+  if (!tb.host.empty()) h2d(mem, tb.host.data(), tb.host.size());
+  char *b = (char *)mem;
+  out->clear();
+  for (auto &p : pp) {
+    FftLaunch L = p.L;
+    for (int d = 0; d < L.ndim; d++) {
+      L.off_in[d] = (const i64 *)(b + p.in_off[d]);
+      L.off_out[d] = (const i64 *)(b + p.out_off[d]);
+    }
+    auto side = [&](FftSide &fs, const PendingSide &ps) {
+      if (fs.uniform) return;
+      for (int d = 0; d < L.ndim; d++) {
+        fs.rank[d] = (const int *)(b + ps.rank[d]);
+        fs.off[d] = (const int *)(b + ps.off[d]);
+      }
+      fs.strides = (const i64 *)(b + ps.strides);
+    };
+    side(L.sin, p.sin);
+    side(L.sout, p.sout);
+    L.ord_in = p.ord_in >= 0 ? (const int *)(b + p.ord_in) : nullptr;
+    L.ord_out = p.ord_out >= 0 ? (const int *)(b + p.ord_out) : nullptr;
+    L.ord_u = p.ord_u >= 0 ? (const int *)(b + p.ord_u) : nullptr;
+    L.twiddle = b + p.tw;
+    out->push_back(L);
+  }
+  return mem;
+}
+
+bool layout_usable(dm_comm_t comm, const char *what, const HostLayout &h, size_t region_elems) {
+  if (!h.offsets_ok) {
+    raise_error(comm, "FFT: the %s decomposition has a negative offset or stride", what);
+    return false;
+  }
+  for (int d = 0; d < h.ndim; d++)
+    for (int g = 0; g < h.L[d]; g++)
+      if (h.rank_part[d][g] >= DM_NO_OWNER) {
+        raise_error(comm, "FFT: coordinate %d of axis %d of the %s decomposition has no owner", g, d, what);
+        return false;
+      }
+  for (int r = 0; r < h.proc_used; r++)
+    if ((size_t)h.slot_extent[r] > region_elems) {
+      raise_error(comm, "FFT: rank %d needs %lld elements of %s, the buffer region holds %zu", r,
+                  (long long)h.slot_extent[r], what, region_elems);
+      return false;
+    }
+  return true;
+}
+
+}  // namespace dm

@@ -1,0 +1,57 @@
+// Building FFT pass programs from layouts (shared by the split plan, the
+// per-axis FFT and the batched transforms).
+#pragma once
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace dm {
+
+// One side (input or output) of a pass: a layout bound to a buffer slot size.
+struct Side {
+  HostLayout h;
+  i64 slot = 0;                         // elements per rank slot of the buffer
+  std::vector<std::vector<i64>> abs;    // absolute element offsets per axis
+  std::vector<std::vector<int>> order;  // per axis: coordinates by increasing offset
+};
+Side make_side(const HostLayout &h, i64 slot);
+
+bool choose_radices(int n, std::vector<int> *out);
+// exp(-2 pi i k / n), k < n, interleaved (re, im), accurate to double rounding
+std::vector<double> roots(int n);
+
+// Host image of the device tables of a program.
+struct TableBuilder {
+  std::vector<char> host;
+  size_t put(const void *src, size_t bytes);
+};
+
+// Table offsets of one side's non-uniform addressing data.
+struct PendingSide {
+  size_t rank[DM_MAX_DIMS];
+  size_t off[DM_MAX_DIMS];
+  size_t strides;
+};
+
+struct PendingPass {
+  FftLaunch L;
+  size_t in_off[DM_MAX_DIMS];
+  size_t out_off[DM_MAX_DIMS];
+  PendingSide sin, sout;
+  long ord_in = -1, ord_out = -1, ord_u = -1, tw = -1;
+};
+
+// Plan the pass over axis t from layout I (buffer in) to layout U (buffer
+// out).  Returns false after raise_error when the axis cannot be done.
+bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, bool inverse, const void *in,
+               void *out, TableBuilder &tb, PendingPass *pp);
+// Upload the tables and resolve table pointers; returns the device block.
+void *finalize_passes(dm_comm_t comm, TableBuilder &tb, std::vector<PendingPass> &pp,
+                      std::vector<FftLaunch> *out);
+
+// Layout validity for FFT use: uniform strides, every coordinate owned,
+// every rank's data inside region_elems.
+bool layout_usable(dm_comm_t comm, const char *what, const HostLayout &h, size_t region_elems);
+
+}  // namespace dm

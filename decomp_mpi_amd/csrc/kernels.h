@@ -1,0 +1,77 @@
+// Launch interface between the host plan compilers and the HIP kernels.
+// Plain structs of device pointers and sizes; no runtime types.
+#pragma once
+#include <cstdint>
+
+#include "internal.h"
+
+namespace dm {
+
+// ---- K1: reshape gather ---------------------------------------------------
+// Every global coordinate g with a valid owner in both layouts is copied:
+//   dst_slot[D(g)] + offset_dst(g)  <-  src_slot[S(g)] + offset_src(g)
+// Coordinates are walked as 2D tiles over (a, b) = (dst fastest axis, src
+// fastest axis) so that reads and writes are both coalesced; the remaining
+// axes are batch.  a == b walks 1D runs along that axis.
+struct ReshapeLaunch {
+  DevLayout src, dst;
+  int ndim;
+  int L[DM_MAX_DIMS];
+  int a, b;
+  int nbatch;
+  int batch_axis[DM_MAX_DIMS];
+  int tiles_a, tiles_b;
+  long long nblocks;
+  const char *src_base;   // slot 0 pointer (already offset by the caller's pointer)
+  char *dst_base;
+  i64 src_slot;           // elements per rank slot
+  i64 dst_slot;
+  int elem_bytes;
+};
+void launch_reshape(const ReshapeLaunch &p, void *stream);
+
+// ---- K2: batched 1D FFT pass -----------------------------------------------
+// Lines along axis t of an ndim array.  With equal strides on every rank
+// (uniform) point g lives at element offset sum_d abs[d][g_d] from the base
+// pointer; otherwise at R*slot + sum_d strides[R][d]*off[d][g_d] with owner
+// R = sum_d rank[d][g_d].  A tile holds T consecutive lines along axis u
+// (u < 0: one line per tile) and goes through LDS.
+struct FftSide {
+  int uniform;
+  int proc_used;
+  i64 slot;                    // elements per rank slot
+  const int *rank[DM_MAX_DIMS];
+  const int *off[DM_MAX_DIMS];
+  const i64 *strides;          // [proc_used * ndim]
+};
+
+struct FftLaunch {
+  FftSide sin, sout;           // non-uniform addressing (used when !uniform)
+  int ndim;
+  int L[DM_MAX_DIMS];
+  int t, u;
+  int T;
+  int N;
+  int nstage;
+  int radix[16];
+  int in_row, out_row;        // 1: walk memory along t; 0: along u
+  int nbatch;
+  int batch_axis[DM_MAX_DIMS];
+  int tiles_u;
+  long long nblocks;
+  const i64 *off_in[DM_MAX_DIMS];
+  const i64 *off_out[DM_MAX_DIMS];
+  const int *ord_in;          // row mode: t coordinates in input address order (or null)
+  const int *ord_out;         // row mode: t coordinates in output address order (or null)
+  const int *ord_u;           // tile line l holds u coordinate ord_u[u0 + l] (or u0 + l)
+  const void *twiddle;        // N entries exp(-2 pi i k / N) in the pass precision
+  int inverse;                // 0 forward, 1 backward (conjugate roots)
+  int dbl;                    // 1: complex double, 0: complex float
+  const void *in;
+  void *out;
+};
+void launch_fft_pass(const FftLaunch &p, void *stream);
+// Largest points per tile the FFT pass kernel holds for an element size.
+int fft_tile_capacity(int dbl);
+
+}  // namespace dm

@@ -1,0 +1,708 @@
+// Split (four-step) parallel FFT: planner in the reference's terms, executed
+// as a device pass program.
+//
+// Reference: split_fft.c (planner 36-77, 249-614, 747-788; executor 622-744;
+// search 78-247), dimscale.c, processor_grid.c.
+//
+// Execution model.  The forward transform maps the start decomposition (in
+// dat) to the output decomposition (work[0:extent), B layout with frequency k
+// of axis d at B-position k / B_d + A_d * (k % B_d)).  Mathematically this is
+// the multi-dimensional DFT with both ends given by layouts; the device
+// program runs one batched FFT pass per axis whose loads and stores go
+// through the layouts' per-axis offset tables, so the reference's
+// start->A reshape, twiddle multiply and A->B reshape need no passes of
+// their own.  Intermediate data live in the output layout itself (any
+// bijection is a valid intermediate layout), which keeps every pass
+// coalesced: the start layout's fastest axis is transformed first (forward)
+// or last (backward), the other passes walk rows of consecutive output
+// positions.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+#include "internal.h"
+#include "kernels.h"
+#include "fft_program.h"
+#include <functional>
+#include "block_decomp.h"
+#include "interleave_decomp.h"
+#include "factor_reverse_decomp.h"
+#include "processor_grid.h"
+#include "logger.h"
+
+namespace dm {
+bool enqueue_reshape(dm_comm_t comm, ReshapePlan plan, const void *src, void *dst, void *stream);
+}
+
+// ---------------------------------------------------------------- processor grid
+// processor_grid.c:24-97 (same enumeration order, score and tie-break).
+
+static void search_grid(GridResult best, int ndim, int mydim, int *grid, const int *tasks, int max_proc,
+                        int proc, int load, int mn, int mx) {
+  int newproc = proc;
+  for (grid[mydim] = 1; grid[mydim] <= max_proc && grid[mydim] <= tasks[mydim] && newproc <= max_proc;
+       grid[mydim]++) {
+    newproc = proc * grid[mydim];
+    if (newproc > max_proc) continue;
+    int lo = tasks[mydim] / grid[mydim];
+    int hi = (tasks[mydim] + grid[mydim] - 1) / grid[mydim];
+    int newload = load * hi;
+    if (mn < lo) lo = mn;
+    if (mx > hi) hi = mx;
+    if (mydim < ndim - 1) {
+      search_grid(best, ndim, mydim + 1, grid, tasks, max_proc, newproc, newload, lo, hi);
+    } else {
+      double ratio = (double)hi / (double)lo;
+      if (newload <= best->load || best->load == 0) {
+        if (best->load == 0 || newload < best->load || ratio < best->asym) {
+          best->procs = newproc;
+          best->load = newload;
+          best->asym = ratio;
+          for (int i = 0; i < ndim; i++) best->length[i] = grid[i];
+        }
+      }
+    }
+  }
+}
+
+extern "C" int *makeProcessorGrid(MPI_Comm comm, int ndim, int tasks[]) {
+  struct grid_result gr;
+  std::vector<int> len(ndim, 1);
+  gr.ndim = ndim;
+  gr.length = len.data();
+  gr.procs = 1;
+  gr.load = 1;
+  gr.asym = 10000.0;
+  int mx = 1;
+  for (int i = 0; i < ndim; i++) {
+    gr.load *= tasks[i];
+    mx = std::max(mx, tasks[i]);
+  }
+  int *grid = (int *)calloc(ndim, sizeof(int));
+  search_grid(&gr, ndim, 0, grid, tasks, comm->world->nranks, 1, 1, mx, 0);
+  for (int i = 0; i < ndim; i++) grid[i] = gr.length[i];
+  return grid;
+}
+
+// split_fft.c:890-926 (any gcd algorithm gives the same value)
+extern "C" unsigned int gcd(unsigned int u, unsigned int v) {
+  while (v != 0) {
+    unsigned int t = u % v;
+    u = v;
+    v = t;
+  }
+  return u;
+}
+
+// ---------------------------------------------------------------- decomps
+
+extern "C" Decomp makeInterleavedDecomp(int ndim, MPI_Comm comm, int A[], int B[], int grid[]) {
+  std::vector<DimDecomp> ax(ndim);
+  for (int i = 0; i < ndim; i++) {
+    DimDecomp blocked = makeBlockDecomp(A[i], grid[i]);
+    ax[i] = makeInterleaveDecomp(blocked, B[i]);
+    freeDimDecomp(blocked);  // the interleave holds its own clone
+  }
+  return makePackedDecomp(comm, ndim, ax.data());
+}
+
+extern "C" Decomp makeBlockedDecomp(int ndim, MPI_Comm comm, int A[], int B[], int grid[]) {
+  std::vector<DimDecomp> ax(ndim);
+  for (int i = 0; i < ndim; i++) {
+    ax[i] = makeBlockDecompWithFactor(A[i] * B[i], A[i], grid[i]);
+    if (ax[i] == nullptr) {
+      for (int j = 0; j < i; j++) freeDimDecomp(ax[j]);
+      return nullptr;
+    }
+  }
+  return makePackedDecomp(comm, ndim, ax.data());
+}
+
+// ---------------------------------------------------------------- dimscale
+// dimscale.c:7-67: multiply every slot by the product of its per-axis
+// factors (walked outer axis to inner); a zero partial product marks an
+// unused slot and skips it.  sign +1 conjugates the product.
+
+static void dim_scale(int dim, Decomp d, _Complex double factor, _Complex double **factors,
+                      _Complex double *data, long long offset, int sign) {
+  AxisDecomp a = d->dims[dim];
+  int count = a->max_local + 1;
+  for (int pos = 0; pos < count; pos++) {
+    long long o = offset + (long long)pos * a->memory_stride;
+    _Complex double f = factor;
+    if (factors != nullptr && factors[dim] != nullptr) f = factor * factors[dim][pos];
+    if (f == 0.0) continue;
+    if (dim == d->ndim - 1) {
+      if (sign == 1) f = __builtin_conj(f);
+      data[o] = f * data[o];
+    } else {
+      dim_scale(dim + 1, d, f, factors, data, o, sign);
+    }
+  }
+}
+
+extern "C" void decomp_scale(Decomp decomp, _Complex double **factors, _Complex double *data, int sign) {
+  dim_scale(0, decomp, 1.0, factors, data, 0, sign);
+}
+
+// split_fft.c:862-889 walks every slot of the local bounding box.  Here the
+// visitor also receives the slot's global coordinates (-1 on axes where the
+// slot holds no coordinate), which the reference leaves unset.
+static void dim_visit(int dim, int *gpos, Decomp d, data_visitor vis, _Complex double *data, long long offset,
+                      const std::vector<std::vector<int>> &inv) {
+  AxisDecomp a = d->dims[dim];
+  int count = a->max_local + 1;
+  for (int pos = 0; pos < count; pos++) {
+    gpos[dim] = inv[dim][pos];
+    long long o = offset + (long long)pos * a->memory_stride;
+    if (dim == d->ndim - 1) vis(d->ndim, gpos, data + o);
+    else dim_visit(dim + 1, gpos, d, vis, data, o, inv);
+  }
+}
+
+extern "C" void visitData(Decomp d, data_visitor vis, _Complex double *dat) {
+  std::vector<std::vector<int>> inv(d->ndim);
+  for (int i = 0; i < d->ndim; i++) {
+    AxisDecomp a = d->dims[i];
+    inv[i].assign(a->max_local + 1, -1);
+    for (int g = 0; g < a->decomp->coord_size; g++)
+      if (isLocal(a, g)) {
+        int o = a->decomp->type_offset(a->decomp, g);
+        if (o >= 0 && o <= a->max_local) inv[i][o] = g;
+      }
+  }
+  std::vector<int> gpos(d->ndim, 0);
+  dim_visit(0, gpos.data(), d, vis, dat, 0, inv);
+}
+
+// ---------------------------------------------------------------- device program
+
+struct dm_fft_program {
+  int inverse = 0;
+  std::vector<dm::FftLaunch> passes;
+  void *mem = nullptr;  // device tables
+};
+
+struct dm_split_ext {
+  int dbl = 1;
+  int ndim = 0;
+  std::vector<int> A, B, L;
+  Decomp start = nullptr;   // caller's (aliased)
+  Decomp out = nullptr;     // B_flipped, or A for single-stage plans (owned by the plan)
+  bool compiled = false;
+  dm_fft_program fwd, bwd;
+};
+
+static void program_release(dm_fft_program &p) {
+  if (p.mem) dm::device_free(p.mem);
+  p.mem = nullptr;
+  p.passes.clear();
+}
+
+// Build one direction's passes.  Forward: dat (start) -> ... -> W0 (output);
+// backward: W0 -> ... -> dat.  Intermediates alternate between W0 and W1.
+static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, const dm::Side &O, void *dat,
+                              void *w0, void *w1, dm_fft_program *prog) {
+  dm_split_ext *x = plan->ext;
+  int nd = x->ndim;
+  std::vector<int> axes;
+  for (int d = 0; d < nd; d++)
+    if (x->L[d] > 1) axes.push_back(d);
+  if (axes.empty()) axes.push_back(0);
+  // forward: start's fastest axis first, then the others from fast to slow
+  int f = S.h.fastest;
+  std::vector<int> order;
+  if (std::find(axes.begin(), axes.end(), f) != axes.end()) order.push_back(f);
+  std::vector<int> rest;
+  for (int d : axes)
+    if (d != f) rest.push_back(d);
+  std::sort(rest.begin(), rest.end(), [&](int a, int b) { return O.h.m[a] < O.h.m[b]; });
+  order.insert(order.end(), rest.begin(), rest.end());
+  if (inverse) std::reverse(order.begin(), order.end());
+  int n = (int)order.size();
+  dm::TableBuilder tb;
+  std::vector<dm::PendingPass> pp(n);
+  void *prev = nullptr;
+  for (int i = 0; i < n; i++) {
+    void *bin, *bout;
+    const dm::Side *si, *so;
+    if (!inverse) {
+      bout = ((n - 1 - i) % 2 == 0) ? w0 : w1;
+      bin = (i == 0) ? dat : prev;
+      si = (i == 0) ? &S : &O;
+      so = &O;
+    } else {
+      bin = (i == 0) ? w0 : prev;
+      bout = (i == n - 1) ? dat : ((i % 2 == 0) ? w1 : w0);
+      si = &O;
+      so = (i == n - 1) ? &S : &O;
+    }
+    if (!dm::plan_pass(plan->comm, *si, *so, order[i], x->dbl, inverse, bin, bout, tb, &pp[i])) return false;
+    prev = bout;
+  }
+  prog->mem = dm::finalize_passes(plan->comm, tb, pp, &prog->passes);
+  prog->inverse = inverse ? 1 : 0;
+  return prog->mem != nullptr;
+}
+
+static bool compile_plan(SplitPlan plan) {
+  dm_split_ext *x = plan->ext;
+  if (x->compiled) return true;
+  dm_comm_t comm = plan->comm;
+  size_t es = x->dbl ? 16 : 8;
+  dm::Buffer bd, bw;
+  size_t od, ow;
+  if (!dm::resolve(plan->src, &bd, &od) || !dm::resolve(plan->work, &bw, &ow)) {
+    raise_error(comm, "split FFT: dat and workspace must point into distributed device buffers "
+                      "(dm_malloc/dm_register)");
+    return false;
+  }
+  if (od % es || ow % es || bd.slot % es || bw.slot % es) {
+    raise_error(comm, "split FFT: buffers must be aligned to the element size");
+    return false;
+  }
+  dm::HostLayout hs = dm::layout_from_decomp(x->start);
+  dm::HostLayout ho = dm::layout_from_decomp(x->out);
+  if (bd.nranks < hs.proc_used || bw.nranks < ho.proc_used) {
+    raise_error(comm, "split FFT: buffers have fewer rank slots than the decompositions use");
+    return false;
+  }
+  size_t ext = (size_t)plan->extent;
+  if (ow + 2 * ext * es > bw.slot) {
+    raise_error(comm, "split FFT: workspace slot of %zu bytes cannot hold 2 x %zu elements", bw.slot - ow, ext);
+    return false;
+  }
+  if (!dm::layout_usable(comm, "start", hs, (bd.slot - od) / es) || !dm::layout_usable(comm, "output", ho, ext))
+    return false;
+  dm::Side S = dm::make_side(hs, (i64)(bd.slot / es));
+  dm::Side O = dm::make_side(ho, (i64)(bw.slot / es));
+  char *dat = bd.base + od;
+  char *w0 = bw.base + ow;
+  char *w1 = w0 + ext * es;
+  if (!compile_direction(plan, false, S, O, dat, w0, w1, &x->fwd)) return false;
+  if (!compile_direction(plan, true, S, O, dat, w0, w1, &x->bwd)) return false;
+  plan->plan_a = &x->fwd;
+  plan->plan_b_back = &x->bwd;
+  x->compiled = true;
+  return true;
+}
+
+// ---------------------------------------------------------------- planner
+
+static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, Decomp start, int ndim, int A[],
+                            int B[], int L[], int grid_a[], int grid_b[], void *dat, int length, void *workspace) {
+  if ((flags & FORCE_USE_FIRST) && !use_first) return nullptr;
+  MPI_Comm comm = start->comm;
+  if (ndim != start->ndim) {
+    raise_error(comm, "buildParallelPlan: ndim %d differs from the start decomposition's %d", ndim, start->ndim);
+    return nullptr;
+  }
+  for (int i = 0; i < ndim; i++)
+    if (A[i] < 1 || B[i] < 1 || A[i] * B[i] != L[i] || L[i] != sizeDimDecomp(start->dims[i]->decomp)) {
+      raise_error(comm, "Invalid factor");
+      return nullptr;
+    }
+  SplitPlan plan = (SplitPlan)calloc(1, sizeof(struct split_fft_plan));
+  plan->comm = comm;
+  plan->src = (fftw_complex *)dat;
+  plan->work = (fftw_complex *)workspace;
+  dm_split_ext *x = new dm_split_ext;
+  plan->ext = x;
+  x->dbl = dbl;
+  x->ndim = ndim;
+  x->A.assign(A, A + ndim);
+  x->B.assign(B, B + ndim);
+  x->L.assign(L, L + ndim);
+  x->start = start;
+
+  // first decomposition keeps the long-distance factor local (split_fft.c:281)
+  plan->A = makeInterleavedDecomp(ndim, comm, A, B, grid_a);
+  if (plan->A == nullptr) {
+    freeSplitPlan(plan);
+    return nullptr;
+  }
+  long long ext = 0;
+  for (long long e : plan->A->ext->extent_of) ext = std::max(ext, e);
+  if (!use_first) {
+    plan->B = makeBlockedDecomp(ndim, comm, A, B, grid_b);
+    if (plan->B == nullptr) {
+      freeSplitPlan(plan);
+      return nullptr;
+    }
+    for (long long e : plan->B->ext->extent_of) ext = std::max(ext, e);
+  }
+  plan->extent = (int)ext;
+  if (2 * ext > length) {
+    logger(comm, "Not enough workspace need=%lld have=%d\n", 2 * ext, length);
+    freeSplitPlan(plan);
+    return nullptr;
+  }
+  if ((flags & PREFER_LARGE_FFT) && !use_first) {
+    // split_fft.c:309-334: reject a common factor in the two blockings
+    for (int i = 0; i < ndim; i++) {
+      InterleaveDecomp ad = (InterleaveDecomp)plan->A->dims[i]->decomp;
+      unsigned a_red = (unsigned)((BlockDecomp)ad->nested)->blocksize;
+      unsigned b_red = (unsigned)(((BlockDecomp)plan->B->dims[i]->decomp)->blocksize / A[i]);
+      if (gcd(a_red, b_red) > 1) {
+        freeSplitPlan(plan);
+        return nullptr;
+      }
+    }
+  }
+  MPI_Datatype base = dbl ? &dm_type_c_double_complex : &dm_type_c_float_complex;
+  plan->start_to_A = makeReshapePlan(comm, base, start, plan->A);
+  plan->A_to_start = makeReshapePlan(comm, base, plan->A, start);
+  if (plan->start_to_A == nullptr || plan->A_to_start == nullptr) {
+    freeSplitPlan(plan);
+    return nullptr;
+  }
+  if (!(use_first || use_second)) {
+    // twiddle tables per axis, indexed by the view rank's local A offset
+    // (split_fft.c:348-420): w = exp(-2 pi i y l / L), y = g % A, l = g / A.
+    int non_trivial = 0;
+    plan->twiddle = (_Complex double **)calloc(ndim, sizeof(_Complex double *));
+    for (int i = 0; i < ndim; i++) {
+      AxisDecomp ax = plan->A->dims[i];
+      plan->twiddle[i] = (_Complex double *)calloc(ax->max_local + 1, sizeof(_Complex double));
+      for (int g = 0; g < L[i]; g++) {
+        if (!isLocal(ax, g)) continue;
+        int local = ax->decomp->type_offset(ax->decomp, g);
+        int y = g % A[i], l = g / A[i];
+        long long rem = ((long long)y * l) % L[i];
+        double re, im;
+        if (rem == 0) re = 1.0, im = 0.0;
+        else if (4 * rem == L[i]) re = 0.0, im = -1.0;
+        else if (2 * rem == L[i]) re = -1.0, im = 0.0;
+        else if (4 * rem == 3LL * L[i]) re = -0.0, im = 1.0;
+        else {
+          double angle = -(2.0 * M_PI) * ((double)((long long)y * l) / (double)L[i]);
+          re = cos(angle);
+          im = sin(angle);
+        }
+        if (re != 1.0 && im != 0.0) non_trivial = 1;  // as split_fft.c:396
+        __real__ plan->twiddle[i][local] = re;
+        __imag__ plan->twiddle[i][local] = im;
+      }
+    }
+    if (!non_trivial) {
+      for (int i = 0; i < ndim; i++) free(plan->twiddle[i]);
+      free(plan->twiddle);
+      plan->twiddle = nullptr;
+    }
+    plan->A_to_B = makeReshapePlan(comm, base, plan->A, plan->B);
+    plan->B_to_A = makeReshapePlan(comm, base, plan->B, plan->A);
+  }
+  if (!use_first) {
+    // frequency k of axis i sits at B-position k / B_i + A_i * (k % B_i):
+    // FactorReverse with the factors in that order (the reference passes
+    // (A_i, B_i), which is the inverse map whenever A_i != B_i).
+    std::vector<AxisDecomp> fl(ndim);
+    for (int i = 0; i < ndim; i++) fl[i] = makeFactorReverseAxisDecomp(B[i], A[i], plan->B->dims[i]);
+    plan->B_flipped = makeDecomp(ndim, fl.data());
+    x->out = plan->B_flipped;
+  } else {
+    x->out = plan->A;
+  }
+  plan->A_factors = (int *)calloc(ndim, sizeof(int));
+  for (int i = 0; i < ndim; i++) plan->A_factors[i] = A[i];
+  plan->times = (double *)calloc(NUM_TIMERS, sizeof(double));
+  return plan;
+}
+
+static SplitPlan propose(int dbl, int flags, Decomp start, int *initial_factor, void *dat, int length,
+                         void *workspace) {
+  int ndim = start->ndim;
+  int use_first = 1, use_second = 1;
+  std::vector<int> A(ndim), B(ndim), L(ndim);
+  for (int i = 0; i < ndim; i++) {
+    B[i] = initial_factor[i];
+    use_second = use_second && B[i] == 1;
+    L[i] = sizeDimDecomp(start->dims[i]->decomp);
+    if (B[i] < 1 || L[i] % B[i] != 0) {
+      raise_error(start->comm, "Invalid factor");
+      return nullptr;
+    }
+    A[i] = L[i] / B[i];
+    use_first = use_first && A[i] == 1;
+  }
+  if (use_second && (flags & PREFER_STAGE1)) return nullptr;
+  int *ga = makeProcessorGrid(start->comm, ndim, A.data());
+  int *gb = makeProcessorGrid(start->comm, ndim, B.data());
+  SplitPlan p = build_plan(dbl, flags, use_first, use_second, start, ndim, A.data(), B.data(), L.data(), ga, gb,
+                           dat, length, workspace);
+  free(ga);
+  free(gb);
+  return p;
+}
+
+static SplitPlan split_plan(int dbl, int slab, Decomp start, void *dat, int length, void *workspace) {
+  int ndim = start->ndim;
+  std::vector<int> B(ndim);
+  for (int i = 0; i < ndim; i++) {
+    int LL = sizeDimDecomp(start->dims[i]->decomp);
+    int AA = 1, BB = LL;
+    if (slab == 1) {
+      if (i == ndim - 1) {
+        BB = 1;
+        AA = LL;
+      }
+    } else {
+      // split_fft.c:590-601: largest divisor not above sqrt(L)
+      for (int q = 1; q < LL; q++)
+        if (LL % q == 0) {
+          int p = LL / q;
+          if (q > AA && q <= p) {
+            AA = q;
+            BB = p;
+          }
+        }
+    }
+    B[i] = BB;
+  }
+  SplitPlan r = propose(dbl, NO_FLAGS, start, B.data(), dat, length, workspace);
+  if (r == nullptr) raise_error(start->comm, "Cannot generate plan");
+  return r;
+}
+
+// ---------------------------------------------------------------- autotuner
+// The reference times every divisor tuple on every grid (split_fft.c:78-247).
+// The device program's cost barely depends on the split (one pass per axis
+// whatever A and B are); what differs is the output layout.  Candidates are
+// ranked by a traffic model (bytes moved per pass, with a penalty for output
+// rows shorter than a 128-byte line), and the best few are timed.
+
+static double model_cost(SplitPlan p) {
+  dm_split_ext *x = p->ext;
+  double N = 1;
+  for (int l : x->L) N *= l;
+  double es = x->dbl ? 16.0 : 8.0;
+  // every pass reads and writes the array once
+  double passes = 0;
+  for (int l : x->L) passes += l > 1;
+  double cost = passes * 2.0 * N * es;
+  // contiguous run of the output's fastest axis on one rank
+  dm::HostLayout ho = dm::layout_from_decomp(x->out);
+  int f = ho.fastest;
+  int nloc = 0;
+  for (int g = 0; g < ho.L[f]; g++) nloc += ho.rank_part[f][g] == 0;
+  double run = std::max(1, nloc) * es;
+  if (run < 128.0) cost *= 128.0 / run;
+  // fewer, larger rank blocks keep the workspace compact
+  cost *= 1.0 + 1e-3 * p->extent / std::max(1.0, N);
+  return cost;
+}
+
+static double time_plan(SplitPlan p, int reps) {
+  if (!compile_plan(p)) return 1e30;
+  void *s = dm::world_stream(p->comm->world);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  auto run = [&]() {
+    for (auto &L : p->ext->fwd.passes) dm::launch_fft_pass(L, s);
+  };
+  run();
+  (void)hipEventRecord(e0, (hipStream_t)s);
+  for (int i = 0; i < reps; i++) run();
+  (void)hipEventRecord(e1, (hipStream_t)s);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+static SplitPlan autotune(int dbl, Decomp start, void *dat, int length, void *workspace) {
+  int ndim = start->ndim;
+  std::vector<int> L(ndim);
+  for (int i = 0; i < ndim; i++) L[i] = sizeDimDecomp(start->dims[i]->decomp);
+  std::vector<std::vector<int>> divs(ndim);
+  for (int i = 0; i < ndim; i++)
+    for (int b = 1; b <= L[i]; b++)
+      if (L[i] % b == 0) divs[i].push_back(b);
+  struct Cand {
+    double cost;
+    SplitPlan plan;
+  };
+  std::vector<Cand> cands;
+  std::vector<int> B(ndim, 0);
+  int em = dm::error_mode();
+  dm_set_error_mode(1);  // candidates that cannot be planned are skipped
+  std::function<void(int)> rec = [&](int d) {
+    if (d == ndim) {
+      bool all_one = true;
+      for (int b : B) all_one = all_one && b == 1;
+      if (all_one) return;  // PREFER_STAGE1: a single second stage is the first-stage plan
+      SplitPlan p = propose(dbl, PREFER_STAGE1, start, B.data(), dat, length, workspace);
+      dm_clear_error();
+      if (p == nullptr) return;
+      cands.push_back({model_cost(p), p});
+      return;
+    }
+    for (int b : divs[d]) {
+      B[d] = b;
+      rec(d + 1);
+    }
+  };
+  // bound the enumeration: beyond a few thousand tuples only balanced splits
+  long long total = 1;
+  for (auto &v : divs) total *= (long long)v.size();
+  if (total <= 4096) {
+    rec(0);
+  } else {
+    for (int i = 0; i < ndim; i++) {
+      std::vector<int> keep;
+      for (int b : divs[i])
+        if ((long long)b * b >= L[i] / 16 && (long long)b * b <= 16LL * L[i]) keep.push_back(b);
+      divs[i] = keep;
+    }
+    rec(0);
+  }
+  dm_set_error_mode(em);
+  if (cands.empty()) return nullptr;
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.cost < b.cost; });
+  size_t keep = std::min<size_t>(cands.size(), 4);
+  SplitPlan best = cands[0].plan;
+  if (dm::device_available() && keep > 1) {
+    double bt = 1e30;
+    for (size_t i = 0; i < keep; i++) {
+      double t = time_plan(cands[i].plan, 3);
+      if (t < bt) {
+        bt = t;
+        best = cands[i].plan;
+      }
+    }
+  }
+  for (auto &c : cands)
+    if (c.plan != best) freeSplitPlan(c.plan);
+  return best;
+}
+
+// ---------------------------------------------------------------- executor
+
+static void run_program(SplitPlan plan, dm_fft_program &prog, int t_first, int t_rest) {
+  dm_world *w = plan->comm->world;
+  void *s = dm::world_stream(w);
+  size_t n = prog.passes.size();
+  bool timed = !w->async;
+  std::vector<hipEvent_t> ev;
+  if (timed) {
+    ev.resize(n + 1);
+    for (auto &e : ev) (void)hipEventCreate(&e);
+    (void)hipEventRecord(ev[0], (hipStream_t)s);
+  }
+  for (size_t i = 0; i < n; i++) {
+    dm::launch_fft_pass(prog.passes[i], s);
+    if (timed) (void)hipEventRecord(ev[i + 1], (hipStream_t)s);
+  }
+  if (!timed) return;
+  (void)hipEventSynchronize(ev[n]);
+  double total = 0;
+  for (size_t i = 0; i < n; i++) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+    plan->times[i == 0 ? t_first : t_rest] += ms * 1e-3;
+    total += ms * 1e-3;
+  }
+  plan->times[ALL] += total;
+  for (auto &e : ev) (void)hipEventDestroy(e);
+}
+
+// split_fft.c:622-744.  Forward: start layout in dat -> output layout in
+// work.  Backward: the exact (unnormalised) inverse, output layout in work ->
+// start layout in dat (the reference's backward path runs A_to_start before
+// its stage-1 inverse and builds that inverse only under MAKE_REVERSE).
+extern "C" void runSplitPlan(SplitPlan plan, int direction) {
+  if (plan == nullptr) {
+    dm::fail(nullptr, "runSplitPlan: NULL plan");
+    return;
+  }
+  if (direction != FFTW_FORWARD && direction != FFTW_BACKWARD) {
+    raise_error(plan->comm, "runSplitPlan: direction must be FFTW_FORWARD or FFTW_BACKWARD");
+    return;
+  }
+  dm::check_device(plan->comm);
+  if (dm::error_pending() && dm::error_mode() == 1) return;
+  if (!compile_plan(plan)) return;
+  if (direction == FFTW_FORWARD) run_program(plan, plan->ext->fwd, FFT_1, FFT_2);
+  else run_program(plan, plan->ext->bwd, FFT_2, FFT_1);
+}
+
+extern "C" Decomp getTransformed(SplitPlan plan) { return plan->B_flipped; }
+
+extern "C" void freeSplitPlan(SplitPlan plan) {
+  if (plan == nullptr) return;
+  int ndim = plan->ext ? plan->ext->ndim : (plan->A ? plan->A->ndim : 0);
+  free(plan->A_factors);
+  freeReshapePlan(plan->start_to_A);
+  freeReshapePlan(plan->A_to_start);
+  freeReshapePlan(plan->A_to_B);
+  freeReshapePlan(plan->B_to_A);
+  if (plan->twiddle != nullptr) {
+    for (int i = 0; i < ndim; i++) free(plan->twiddle[i]);
+    free(plan->twiddle);
+  }
+  if (plan->ext) {
+    program_release(plan->ext->fwd);
+    program_release(plan->ext->bwd);
+    delete plan->ext;
+  }
+  freeDecomp(plan->B_flipped);
+  freeDecomp(plan->B);
+  freeDecomp(plan->A);
+  free(plan->times);
+  free(plan);
+}
+
+// ---------------------------------------------------------------- C entry points
+
+extern "C" SplitPlan buildParallelPlan(int flags, int use_first, int use_second, Decomp start, int ndim, int A[],
+                                       int B[], int L[], int a_grid[], int b_grid[], fftw_complex *dat, int length,
+                                       fftw_complex *workspace) {
+  return build_plan(1, flags, use_first, use_second, start, ndim, A, B, L, a_grid, b_grid, dat, length,
+                    workspace);
+}
+extern "C" SplitPlan proposeParallelPlan(int flags, Decomp start, int *initial_factor, fftw_complex *dat,
+                                         int length, fftw_complex *workspace) {
+  return propose(1, flags, start, initial_factor, dat, length, workspace);
+}
+extern "C" SplitPlan autotuneParallelPlan(Decomp start, fftw_complex *dat, int length, fftw_complex *workspace) {
+  return autotune(1, start, dat, length, workspace);
+}
+extern "C" SplitPlan makeSplitPlan(int slab, Decomp start, fftw_complex *dat, int length,
+                                   fftw_complex *workspace) {
+  return split_plan(1, slab, start, dat, length, workspace);
+}
+extern "C" SplitPlan makeParallelPlan(Decomp start, int *initial_factor, fftw_complex *dat, int length,
+                                      fftw_complex *workspace) {
+  return propose(1, NO_FLAGS, start, initial_factor, dat, length, workspace);
+}
+
+extern "C" SplitPlan buildParallelPlanf(int flags, int use_first, int use_second, Decomp start, int ndim, int A[],
+                                        int B[], int L[], int a_grid[], int b_grid[], fftwf_complex *dat,
+                                        int length, fftwf_complex *workspace) {
+  return build_plan(0, flags, use_first, use_second, start, ndim, A, B, L, a_grid, b_grid, dat, length,
+                    workspace);
+}
+extern "C" SplitPlan proposeParallelPlanf(int flags, Decomp start, int *initial_factor, fftwf_complex *dat,
+                                          int length, fftwf_complex *workspace) {
+  return propose(0, flags, start, initial_factor, dat, length, workspace);
+}
+extern "C" SplitPlan autotuneParallelPlanf(Decomp start, fftwf_complex *dat, int length,
+                                           fftwf_complex *workspace) {
+  return autotune(0, start, dat, length, workspace);
+}
+extern "C" SplitPlan makeSplitPlanf(int slab, Decomp start, fftwf_complex *dat, int length,
+                                    fftwf_complex *workspace) {
+  return split_plan(0, slab, start, dat, length, workspace);
+}
+extern "C" SplitPlan makeParallelPlanf(Decomp start, int *initial_factor, fftwf_complex *dat, int length,
+                                       fftwf_complex *workspace) {
+  return propose(0, NO_FLAGS, start, initial_factor, dat, length, workspace);
+}

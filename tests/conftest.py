@@ -1,0 +1,15 @@
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+    lib = os.path.join(ROOT, "decomp_mpi_amd", "libdecomp_mpi_amd.so")
+    orc = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(lib) or not os.path.exists(orc):
+        import __graft_entry__
+        __graft_entry__.build()
