@@ -90,7 +90,7 @@ _SIGS = {
     "dm_set_stream": (I, [P, P]), "dm_set_error_mode": (None, [I]),
     "dm_last_error": (ctypes.c_char_p, []), "dm_clear_error": (None, []),
     "dm_plan_axis_fft": (P, [P, P, I, I, I]), "dm_run_axis_fft": (I, [P, P, P]),
-    "dm_free_axis_fft": (None, [P]),
+    "dm_free_axis_fft": (None, [P]), "dm_copy_bench": (D, [P, P, P, SZ, I]),
     # index.h
     "makeIndex": (P, []), "freeIndex": (None, [P]), "addIndex": (None, [P, I]),
     # descriptors

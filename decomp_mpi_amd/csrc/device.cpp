@@ -85,3 +85,28 @@ extern "C" int dm_fill(void *dst, int value, size_t bytes_per_rank) {
   dm::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   return DM_SUCCESS;
 }
+
+namespace dm {
+void launch_copy16(const void *src, void *dst, long long n16, void *stream);
+}
+
+// Plain device-to-device copy of `bytes` (a multiple of 16) on the world's
+// stream, timed with HIP events; returns milliseconds (negative on error).
+extern "C" double dm_copy_bench(MPI_Comm comm, void *dst, const void *src, size_t bytes, int reps) {
+  if (comm == nullptr || bytes % 16 != 0 || reps < 1) return -1.0;
+  dm::check_device(comm);
+  hipStream_t s = (hipStream_t)dm::world_stream(comm->world);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  dm::launch_copy16(src, dst, (long long)(bytes / 16), s);
+  (void)hipEventRecord(e0, s);
+  for (int i = 0; i < reps; i++) dm::launch_copy16(src, dst, (long long)(bytes / 16), s);
+  (void)hipEventRecord(e1, s);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}

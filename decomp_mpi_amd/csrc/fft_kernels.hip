@@ -10,6 +10,10 @@
 // T lines of the tile (column mode), so that global accesses coalesce.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace dm {
@@ -209,13 +213,24 @@ struct Pass {
   StageInfo st[16];
   unsigned long long m_n, m_t;  // magics for N and T
   int ls;                        // LDS line stride (complex elements)
+  int fuse_in;                   // first stage runs on the loaded registers
+  int fuse_out;                  // last stage stores straight from registers
+  int xcd_pair;                  // tiles sharing 128-byte rows, run back to back on one XCD
+  int tw_lds;                    // twiddle table copied to LDS
 };
 
-__device__ __forceinline__ int lds_index(int line, int x, int ls) { return line * ls + x + (x >> 4); }
+// LDS placement of point x of tile line `line`: one pad element every 16
+// points and an odd line stride in elements, so a column walk over
+// consecutive lines and strided Stockham writes spread over the banks.
+// (The tile plus a 512-point twiddle table stays under 80 KiB: two
+// workgroups per CU.)
+__host__ __device__ __forceinline__ int lds_pad(int x) { return x + (x >> 4); }
+__host__ __device__ __forceinline__ int lds_line(int n) { return n + (n >> 4) + 1; }
+__device__ __forceinline__ int lds_index(int line, int x, int ls) { return line * ls + lds_pad(x); }
 
-template <typename R, bool INV, int RADIX, int CAP>
+template <typename R, bool INV, int RADIX, int CAP, int NTH>
 __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx<R> *tw) {
-  constexpr int K = (CAP / RADIX + NT - 1) / NT;
+  constexpr int K = (CAP / RADIX + NTH - 1) / NTH;
   const StageInfo &si = P.st[s];
   const int nr = si.nr, ns = si.ns;
   const int nb = P.p.T * nr;
@@ -223,7 +238,7 @@ __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx
   int dst[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    int b = threadIdx.x + k * NT;
+    int b = threadIdx.x + k * NTH;
     dst[k] = -1;
     if (b < nb) {
       int l = fdiv(b, si.m_nr);
@@ -234,7 +249,7 @@ __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx
 #pragma unroll
       for (int r = 0; r < RADIX; r++) {
         int x = j + r * nr;
-        v[k][r] = buf[base + x + (x >> 4)];
+        v[k][r] = buf[base + lds_pad(x)];
       }
       if (ns > 1) {
 #pragma unroll
@@ -259,7 +274,7 @@ __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx
 #pragma unroll
       for (int r = 0; r < RADIX; r++) {
         int x = idxd + r * ns;
-        buf[base + x + (x >> 4)] = v[k][r];
+        buf[base + lds_pad(x)] = v[k][r];
       }
     }
   }
@@ -319,39 +334,194 @@ struct LineAddr<false> {
   }
 };
 
-template <typename R, bool INV, int CAP, bool UIN, bool UOUT>
-__global__ __launch_bounds__(NT) void fft_pass_kernel(Pass P) {
-  extern __shared__ char smem_raw[];
-  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
-  const FftLaunch &p = P.p;
-  const int N = p.N, T = p.T;
-
-  // block -> (tile along u, batch coordinates)
-  long long bid = blockIdx.x;
-  int tu = (int)(bid % p.tiles_u);
-  bid /= p.tiles_u;
+// Tile t -> (tile along u, batch coordinates); fills the per-line address
+// data of both sides for that tile (threads l < T).
+template <bool UIN, bool UOUT>
+__device__ __forceinline__ int setup_lines(const FftLaunch &p, long long tile, LineAddr<UIN> &lin,
+                                           LineAddr<UOUT> &lout, int T, int NTHR) {
+  int tu = (int)(tile % p.tiles_u);
+  const int u0 = tu * T;
+  const int nlines = p.u >= 0 ? min(T, p.L[p.u] - u0) : 1;
+  if constexpr (UIN && UOUT) {
+    if (p.line_in_tab != nullptr) {  // precomputed line bases: one load each
+      for (int l = threadIdx.x; l < T; l += NTHR) {
+        lin.base[l] = p.line_in_tab[tile * T + l];
+        lout.base[l] = p.line_out_tab[tile * T + l];
+      }
+      return nlines;
+    }
+  }
+  long long bid = tile / p.tiles_u;
   int gb[DM_MAX_DIMS];
+  for (int d = 0; d < DM_MAX_DIMS; d++) gb[d] = 0;
   for (int k = p.nbatch - 1; k >= 0; k--) {
     int d = p.batch_axis[k];
     gb[d] = (int)(bid % p.L[d]);
     bid /= p.L[d];
   }
-  const int u0 = tu * T;
-  const int nlines = p.u >= 0 ? min(T, p.L[p.u] - u0) : 1;
+  for (int l = threadIdx.x; l < T; l += NTHR) {
+    int g[DM_MAX_DIMS];
+    for (int d = 0; d < p.ndim; d++) g[d] = gb[d];
+    g[p.t] = 0;
+    if (p.u >= 0) g[p.u] = (l < nlines) ? (p.ord_u ? p.ord_u[u0 + l] : u0 + l) : 0;
+    if constexpr (UIN) lin.set_line(p, p.off_in, g, l);
+    else lin.set_line(p, p.sin, g, l);
+    if constexpr (UOUT) lout.set_line(p, p.off_out, g, l);
+    else lout.set_line(p, p.sout, g, l);
+  }
+  return nlines;
+}
+
+// (line, point) of tile element e for a row walk (along the line) or a
+// column walk (across the tile's lines); e must be below T * N.
+__device__ __forceinline__ void element(const Pass &P, int row, const int *ord, int e, int &l, int &x) {
+  if (row) {
+    l = fdiv(e, P.m_n);
+    int i = e - l * P.p.N;
+    x = ord ? ord[i] : i;
+  } else {
+    x = fdiv(e, P.m_t);
+    l = e - x * P.p.T;
+  }
+}
+
+// Butterfly q of this thread in a stage with nr butterflies per line:
+// column walks put consecutive threads on consecutive lines, row walks on
+// consecutive butterflies of a line, so each input/output index r of the
+// butterfly is one coalesced access across the wave.
+__device__ __forceinline__ void butterfly_of(int bi, int row, int nr, unsigned long long m_nr, int T,
+                                             unsigned long long m_t, int &l, int &j) {
+  if (row) {
+    l = fdiv(bi, m_nr);
+    j = bi - l * nr;
+  } else {
+    j = fdiv(bi, m_t);
+    l = bi - j * T;
+  }
+}
+
+// First stage on the loaded values: each thread loads whole butterflies
+// (points j + nr*r of line l), transforms them in registers (no twiddle in
+// the first stage) and writes the outputs to their Stockham places in LDS.
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool UIN>
+__device__ __forceinline__ void load_first_stage(cx<R> *buf, const Pass &P, const LineAddr<UIN> &lin,
+                                                 const cx<R> *in, const i64 *inT, int nl) {
+  constexpr int K = (CAP / NTH) / RADIX;
+  const FftLaunch &p = P.p;
+  const StageInfo &si = P.st[0];
+  const int nr = si.nr, nb = p.T * nr;
+  i64 a[K][RADIX];
+  int lk[K], jk[K];
+  bool ok[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int bi = threadIdx.x + q * NTH;
+    int bc = min(bi, nb - 1);
+    butterfly_of(bc, p.in_row, nr, si.m_nr, p.T, P.m_t, lk[q], jk[q]);
+    ok[q] = bi < nb && lk[q] < nl;
+    int ls = ok[q] ? lk[q] : 0;
+#pragma unroll
+    for (int r = 0; r < RADIX; r++) a[q][r] = lin.at(p, p.sin, inT, ls, jk[q] + r * nr);
+  }
+  cx<R> v[K][RADIX];
+#pragma unroll
+  for (int q = 0; q < K; q++)
+#pragma unroll
+    for (int r = 0; r < RADIX; r++) v[q][r] = in[a[q][r]];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    if (!ok[q])
+#pragma unroll
+      for (int r = 0; r < RADIX; r++) v[q][r] = {0, 0};
+    dft<R, INV, RADIX>(v[q]);
+    if (threadIdx.x + q * NTH < nb) {
+      int base = lk[q] * P.ls, idxd = jk[q] * RADIX;
+#pragma unroll
+      for (int r = 0; r < RADIX; r++) buf[base + lds_pad(idxd + r)] = v[q][r];
+    }
+  }
+}
+
+// Last stage straight to memory: read butterfly inputs from LDS, twiddle,
+// transform, and store outputs j + nr*r of line l.
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool UOUT>
+__device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P, const LineAddr<UOUT> &lout,
+                                                 cx<R> *out, const i64 *outT, const cx<R> *tw, int nl) {
+  constexpr int K = (CAP / NTH) / RADIX;
+  const FftLaunch &p = P.p;
+  const StageInfo &si = P.st[p.nstage - 1];
+  const int nr = si.nr, nb = p.T * nr;
+  cx<R> v[K][RADIX];
+  i64 a[K][RADIX];
+  bool ok[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int bi = threadIdx.x + q * NTH;
+    int bc = min(bi, nb - 1);
+    int l, j;
+    butterfly_of(bc, p.out_row, nr, si.m_nr, p.T, P.m_t, l, j);
+    ok[q] = bi < nb && l < nl;
+    int base = l * P.ls;
+#pragma unroll
+    for (int r = 0; r < RADIX; r++) v[q][r] = buf[base + lds_pad(j + r * nr)];
+    // in the last stage ns = nr, so j % ns = j and the twiddle stride is 1
+#pragma unroll
+    for (int r = 1; r < RADIX; r++) {
+      cx<R> w = tw[j * r];
+      if (INV) w.y = -w.y;
+      v[q][r] = cmul(v[q][r], w);
+    }
+    dft<R, INV, RADIX>(v[q]);
+    int ls = ok[q] ? l : 0;
+#pragma unroll
+    for (int r = 0; r < RADIX; r++) a[q][r] = lout.at(p, p.sout, outT, ls, j + r * nr);
+  }
+#pragma unroll
+  for (int q = 0; q < K; q++)
+    if (ok[q])
+#pragma unroll
+      for (int r = 0; r < RADIX; r++) out[a[q][r]] = v[q][r];
+}
+
+// One tile per workgroup.  Loads and stores run in three branch-free
+// phases -- index lookups and addresses, then all data accesses back to
+// back, then the LDS side -- because vector memory operations complete in
+// order: a lookup issued after a data load would make the data load's
+// latency part of every later address computation.
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT>
+__global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  const FftLaunch &p = P.p;
+  const int T = p.T;
+  constexpr int V = CAP / NTH;
+  const int tot = T * p.N;
 
   char *lmem = smem_raw + (size_t)T * P.ls * sizeof(cx<R>);
   LineAddr<UIN> lin;
   LineAddr<UOUT> lout;
   lin.init(lmem, T, p.ndim);
   lout.init(lmem + LineAddr<UIN>::bytes(T, p.ndim), T, p.ndim);
-  for (int l = threadIdx.x; l < T; l += NT) {
-    int g[DM_MAX_DIMS];
-    for (int d = 0; d < p.ndim; d++) g[d] = (d == p.u || d == p.t) ? 0 : gb[d];
-    if (p.u >= 0) g[p.u] = (l < nlines) ? (p.ord_u ? p.ord_u[u0 + l] : u0 + l) : 0;
-    if constexpr (UIN) lin.set_line(p, p.off_in, g, l);
-    else lin.set_line(p, p.sin, g, l);
-    if constexpr (UOUT) lout.set_line(p, p.off_out, g, l);
-    else lout.set_line(p, p.sout, g, l);
+  // Workgroups are dispatched round-robin over the 8 XCDs.  When a tile's
+  // rows are shorter than a 128-byte line, the xcd_pair tiles that share
+  // those lines are given to consecutive workgroups of one XCD, so its L2
+  // merges the partial lines instead of each XCD fetching whole lines.
+  long long tile = blockIdx.x;
+  if (P.xcd_pair > 1) {
+    const long long G = 8LL * P.xcd_pair;
+    if (tile < (p.nblocks / G) * G) {
+      long long r = tile % G;
+      tile = tile - r + (r % 8) * P.xcd_pair + r / 8;
+    }
+  }
+  const int nl = setup_lines<UIN, UOUT>(p, tile, lin, lout, T, NTH);
+  // twiddle table in LDS after the line data
+  size_t tw_at = (size_t)T * P.ls * sizeof(cx<R>) + LineAddr<UIN>::bytes(T, p.ndim) +
+                 LineAddr<UOUT>::bytes(T, p.ndim);
+  cx<R> *twl = reinterpret_cast<cx<R> *>(smem_raw + ((tw_at + 15) & ~(size_t)15));
+  if (P.tw_lds) {
+    const cx<R> *twg = reinterpret_cast<const cx<R> *>(p.twiddle);
+    for (int i = threadIdx.x; i < p.N; i += NTH) twl[i] = twg[i];
   }
   __syncthreads();
 
@@ -359,88 +529,187 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(Pass P) {
   const i64 *outT = p.off_out[p.t];
   const cx<R> *in = reinterpret_cast<const cx<R> *>(p.in);
   cx<R> *out = reinterpret_cast<cx<R> *>(p.out);
-  const cx<R> *tw = reinterpret_cast<const cx<R> *>(p.twiddle);
-  const int tot = T * N;
+  const cx<R> *tw = P.tw_lds ? twl : reinterpret_cast<const cx<R> *>(p.twiddle);
 
-  for (int e = threadIdx.x; e < tot; e += NT) {
-    int l, x;
-    if (p.in_row) {
-      l = fdiv(e, P.m_n);
-      int i = e - l * N;
-      x = p.ord_in ? p.ord_in[i] : i;
-    } else {
-      x = fdiv(e, P.m_t);
-      l = e - x * T;
+  int s0 = 0, s1 = p.nstage;
+  if (P.fuse_in) {
+    switch (p.radix[0]) {
+      case 2: if constexpr ((CAP / NTH) % 2 == 0) load_first_stage<R, INV, 2, CAP, NTH, UIN>(buf, P, lin, in, inT, nl); break;
+      case 4: if constexpr ((CAP / NTH) % 4 == 0) load_first_stage<R, INV, 4, CAP, NTH, UIN>(buf, P, lin, in, inT, nl); break;
+      case 8: if constexpr ((CAP / NTH) % 8 == 0) load_first_stage<R, INV, 8, CAP, NTH, UIN>(buf, P, lin, in, inT, nl); break;
+      case 16: if constexpr ((CAP / NTH) % 16 == 0) load_first_stage<R, INV, 16, CAP, NTH, UIN>(buf, P, lin, in, inT, nl); break;
+      default: break;
     }
-    cx<R> v = {0, 0};
-    if (l < nlines) v = in[lin.at(p, p.sin, inT, l, x)];
-    buf[lds_index(l, x, P.ls)] = v;
+    s0 = 1;
+  } else {
+    i64 a[V];
+    int sl[V];
+    bool ok[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+      int e = threadIdx.x + k * NTH;
+      int ec = min(e, tot - 1);
+      int l, x;
+      element(P, p.in_row, p.ord_in, ec, l, x);
+      ok[k] = e < tot && l < nl;
+      sl[k] = e < tot ? lds_index(l, x, P.ls) : -1;
+      a[k] = lin.at(p, p.sin, inT, ok[k] ? l : 0, x);
+    }
+    cx<R> v[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) v[k] = in[a[k]];
+#pragma unroll
+    for (int k = 0; k < V; k++)
+      if (sl[k] >= 0) buf[sl[k]] = ok[k] ? v[k] : cx<R>{0, 0};
   }
   __syncthreads();
+  if (P.fuse_out) s1 = p.nstage - 1;
 
-  for (int s = 0; s < p.nstage; s++) {
+  for (int s = s0; s < s1; s++) {
     switch (p.radix[s]) {
-      case 2: stage<R, INV, 2, CAP>(buf, P, s, tw); break;
-      case 3: stage<R, INV, 3, CAP>(buf, P, s, tw); break;
-      case 4: stage<R, INV, 4, CAP>(buf, P, s, tw); break;
-      case 5: stage<R, INV, 5, CAP>(buf, P, s, tw); break;
-      case 7: stage<R, INV, 7, CAP>(buf, P, s, tw); break;
-      case 8: stage<R, INV, 8, CAP>(buf, P, s, tw); break;
-      case 11: stage<R, INV, 11, CAP>(buf, P, s, tw); break;
-      case 13: stage<R, INV, 13, CAP>(buf, P, s, tw); break;
-      case 16: stage<R, INV, 16, CAP>(buf, P, s, tw); break;
+      case 2: stage<R, INV, 2, CAP, NTH>(buf, P, s, tw); break;
+      case 3: if constexpr (!POW2) stage<R, INV, 3, CAP, NTH>(buf, P, s, tw); break;
+      case 4: stage<R, INV, 4, CAP, NTH>(buf, P, s, tw); break;
+      case 5: if constexpr (!POW2) stage<R, INV, 5, CAP, NTH>(buf, P, s, tw); break;
+      case 7: if constexpr (!POW2) stage<R, INV, 7, CAP, NTH>(buf, P, s, tw); break;
+      case 8: stage<R, INV, 8, CAP, NTH>(buf, P, s, tw); break;
+      case 11: if constexpr (!POW2) stage<R, INV, 11, CAP, NTH>(buf, P, s, tw); break;
+      case 13: if constexpr (!POW2) stage<R, INV, 13, CAP, NTH>(buf, P, s, tw); break;
+      case 16: stage<R, INV, 16, CAP, NTH>(buf, P, s, tw); break;
       default: break;
     }
   }
 
-  for (int e = threadIdx.x; e < tot; e += NT) {
-    int l, x;
-    if (p.out_row) {
-      l = fdiv(e, P.m_n);
-      int i = e - l * N;
-      x = p.ord_out ? p.ord_out[i] : i;
-    } else {
-      x = fdiv(e, P.m_t);
-      l = e - x * T;
+  if (P.fuse_out) {
+    switch (p.radix[p.nstage - 1]) {
+      case 2: if constexpr ((CAP / NTH) % 2 == 0) last_stage_store<R, INV, 2, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+      case 4: if constexpr ((CAP / NTH) % 4 == 0) last_stage_store<R, INV, 4, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+      case 8: if constexpr ((CAP / NTH) % 8 == 0) last_stage_store<R, INV, 8, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+      case 16: if constexpr ((CAP / NTH) % 16 == 0) last_stage_store<R, INV, 16, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+      default: break;
     }
-    if (l < nlines) out[lout.at(p, p.sout, outT, l, x)] = buf[lds_index(l, x, P.ls)];
+  } else {
+    i64 a[V];
+    int sl[V];
+    bool ok[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+      int e = threadIdx.x + k * NTH;
+      int ec = min(e, tot - 1);
+      int l, x;
+      element(P, p.out_row, p.ord_out, ec, l, x);
+      ok[k] = e < tot && l < nl;
+      sl[k] = lds_index(l, x, P.ls);
+      a[k] = lout.at(p, p.sout, outT, ok[k] ? l : 0, x);
+    }
+    cx<R> w[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) w[k] = buf[sl[k]];
+#pragma unroll
+    for (int k = 0; k < V; k++)
+      if (ok[k]) out[a[k]] = w[k];
   }
 }
 
 unsigned long long magic_for(int d) { return (0x100000000ULL / (unsigned long long)d) + 1ULL; }
 
-constexpr int CAP_D = 4096;  // complex double points per tile
-constexpr int CAP_F = 8192;  // complex float points per tile
+// Big tiles: 8 lines of 512 complex doubles (16 of complex floats) on 512
+// threads; two such workgroups share a CU.  A 1024-thread tile of 16 lines
+// (256-byte column rows) measured slower: one workgroup per CU hides less
+// latency than the wider rows gain.
+constexpr int CAP_D = 4096;  // complex double points per big tile
+constexpr int CAP_F = 8192;  // complex float points per big tile
+constexpr int NTH_BIG = 512;
 
-template <typename R, bool INV, int CAP, bool UIN, bool UOUT>
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT>
 void launch_typed(const Pass &P, size_t lds, hipStream_t s) {
   static bool attr_set = false;
+  static int cus = 0;
+  auto k = fft_pass_kernel<R, INV, CAP, NTH, POW2, UIN, UOUT>;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)fft_pass_kernel<R, INV, CAP, UIN, UOUT>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     attr_set = true;
   }
-  hipLaunchKernelGGL((fft_pass_kernel<R, INV, CAP, UIN, UOUT>), dim3((unsigned)P.p.nblocks), dim3(NT), lds, s, P);
+  // Persistent grid: as many workgroups as fit on the device at once.  The
+  // runtime's occupancy query assumes 64 KiB of LDS per CU; gfx950 has 160
+  // KiB, so residency is computed here from LDS and VGPR use (512 VGPRs per
+  // SIMD lane, at most 8 waves per SIMD, 4 SIMDs per CU).
+  static int vgprs = -1;
+  if (vgprs < 0) {
+    hipFuncAttributes fa;
+    vgprs = (hipFuncGetAttributes(&fa, (const void *)k) == hipSuccess && fa.numRegs > 0) ? fa.numRegs : 128;
+  }
+  int waves_per_wg = NTH / 64;
+  int lds_fit = (int)((160 * 1024) / std::max<size_t>(lds, 1));
+  int waves_per_simd = std::min(8, 512 / (((vgprs + 7) / 8) * 8));
+  int vgpr_fit = (waves_per_simd * 4) / waves_per_wg;
+  int per_cu = std::max(1, std::min(lds_fit, vgpr_fit));
+  long long grid = P.p.nblocks;  // one tile per workgroup
+  static int debug = -1;
+  if (debug < 0) debug = getenv("DM_DEBUG") != nullptr;
+  if (debug)
+    fprintf(stderr, "fft pass N=%d T=%d tiles=%lld grid=%lld per_cu=%d cus=%d lds=%zu\n", P.p.N, P.p.T,
+            P.p.nblocks, grid, per_cu, cus, lds);
+  hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(NTH), lds, s, P);
 }
 
-template <typename R, bool INV, int CAP>
+template <typename R, bool INV, int CAP, int NTH, bool POW2>
 void launch_layouts(const Pass &P, size_t lds, hipStream_t s) {
   bool ui = P.p.sin.uniform, uo = P.p.sout.uniform;
-  if (ui && uo) launch_typed<R, INV, CAP, true, true>(P, lds, s);
-  else if (ui) launch_typed<R, INV, CAP, true, false>(P, lds, s);
-  else if (uo) launch_typed<R, INV, CAP, false, true>(P, lds, s);
-  else launch_typed<R, INV, CAP, false, false>(P, lds, s);
+  if (ui && uo) launch_typed<R, INV, CAP, NTH, POW2, true, true>(P, lds, s);
+  else if (ui) launch_typed<R, INV, CAP, NTH, POW2, true, false>(P, lds, s);
+  else if (uo) launch_typed<R, INV, CAP, NTH, POW2, false, true>(P, lds, s);
+  else launch_typed<R, INV, CAP, NTH, POW2, false, false>(P, lds, s);
+}
+
+// Power-of-two radix schedules get a kernel without the prime butterflies
+// (fewer registers); the thread count keeps 8 (FP64) or 16 (FP32) points
+// per thread in the load and store phases.
+// Two tile sizes with the same points per thread: CAP points on 512
+// threads, or CAP/2 points on 256 threads (half the LDS, twice the
+// workgroups per CU).
+template <typename R, bool INV, int CAP>
+void launch_radices(const Pass &P, size_t lds, hipStream_t s) {
+  bool pow2 = true;
+  for (int i = 0; i < P.p.nstage; i++) pow2 = pow2 && (P.p.radix[i] & (P.p.radix[i] - 1)) == 0;
+  bool small = P.p.T * P.p.N <= CAP / 2;
+  if (small) {
+    if (pow2) launch_layouts<R, INV, CAP / 2, NTH_BIG / 2, true>(P, lds, s);
+    else launch_layouts<R, INV, CAP / 2, NTH_BIG / 2, false>(P, lds, s);
+  } else {
+    if (pow2) launch_layouts<R, INV, CAP, NTH_BIG, true>(P, lds, s);
+    else launch_layouts<R, INV, CAP, NTH_BIG, false>(P, lds, s);
+  }
 }
 
 }  // namespace
 
-int fft_tile_capacity(int dbl) { return dbl ? CAP_D : CAP_F; }
+// Points per tile the planner should use.  Column walks need the big tile
+// (T * element = 128-byte rows); pure row walks use the half tile (twice the
+// workgroups per CU).  DM_FFT_TILE=big|small forces one size.
+int fft_tile_capacity(int dbl, int column) {
+  static int force = -2;
+  if (force == -2) {
+    const char *e = getenv("DM_FFT_TILE");
+    force = e == nullptr ? -1 : (e[0] == 'b' ? 0 : 1);
+  }
+  int cap = dbl ? CAP_D : CAP_F;
+  bool small = force >= 0 ? force == 1 : !column;
+  return small ? cap / 2 : cap;
+}
 
 size_t fft_lds_bytes(const FftLaunch &p) {
   size_t elem = p.dbl ? 16 : 8;
-  size_t ls = (size_t)p.N + (p.N >> 4);
+  size_t ls = (size_t)lds_line(p.N);
   auto side = [&](int uniform) { return uniform ? (size_t)p.T * 8 : (size_t)p.T * 4 * (1 + p.ndim); };
-  return (size_t)p.T * ls * elem + side(p.sin.uniform) + side(p.sout.uniform);
+  // tile data + per-line address data of both sides
+  size_t b = (size_t)p.T * ls * elem + side(p.sin.uniform) + side(p.sout.uniform);
+  b = (b + 15) & ~(size_t)15;
+  if ((size_t)p.N * elem <= 16384) b += (size_t)p.N * elem;  // twiddle table
+  return b;
 }
 
 void launch_fft_pass(const FftLaunch &p, void *stream) {
@@ -448,7 +717,27 @@ void launch_fft_pass(const FftLaunch &p, void *stream) {
   P.p = p;
   P.m_n = magic_for(p.N);
   P.m_t = magic_for(p.T);
-  P.ls = p.N + (p.N >> 4);
+  P.ls = lds_line(p.N);
+  {
+    // Fused first/last stages need whole butterflies per thread (points per
+    // thread a multiple of the radix) and an unpermuted walk along the line
+    // in row mode (a permuted one goes through LDS to stay coalesced).
+    const int V = (p.dbl ? CAP_D : CAP_F) / NTH_BIG;
+    auto pow2 = [](int r) { return r == 2 || r == 4 || r == 8 || r == 16; };
+    int r0 = p.radix[0], rl = p.radix[p.nstage > 0 ? p.nstage - 1 : 0];
+    P.fuse_in = p.nstage >= 1 && pow2(r0) && V % r0 == 0 && (!p.in_row || p.ord_in == nullptr);
+    P.fuse_out = p.nstage >= 2 && pow2(rl) && V % rl == 0 && (!p.out_row || p.ord_out == nullptr);
+    static int off = -1;
+    if (off < 0) off = getenv("DM_NO_FUSE") != nullptr;
+    if (off) P.fuse_in = P.fuse_out = 0;
+    // column walks with rows shorter than 128 bytes: pair the tiles that
+    // share lines on one XCD
+    int elem = p.dbl ? 16 : 8;
+    int row_bytes = p.T * elem;
+    bool col = !p.in_row || !p.out_row;
+    P.xcd_pair = (col && p.u >= 0 && row_bytes < 128) ? 128 / row_bytes : 1;
+    P.tw_lds = (size_t)p.N * elem <= 16384;
+  }
   int ns = 1;
   for (int s = 0; s < p.nstage; s++) {
     int r = p.radix[s];
@@ -466,11 +755,11 @@ void launch_fft_pass(const FftLaunch &p, void *stream) {
   size_t lds = fft_lds_bytes(p);
   hipStream_t s = (hipStream_t)stream;
   if (p.dbl) {
-    if (p.inverse) launch_layouts<double, true, CAP_D>(P, lds, s);
-    else launch_layouts<double, false, CAP_D>(P, lds, s);
+    if (p.inverse) launch_radices<double, true, CAP_D>(P, lds, s);
+    else launch_radices<double, false, CAP_D>(P, lds, s);
   } else {
-    if (p.inverse) launch_layouts<float, true, CAP_F>(P, lds, s);
-    else launch_layouts<float, false, CAP_F>(P, lds, s);
+    if (p.inverse) launch_radices<float, true, CAP_F>(P, lds, s);
+    else launch_radices<float, false, CAP_F>(P, lds, s);
   }
 }
 

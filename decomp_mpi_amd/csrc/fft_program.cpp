@@ -103,7 +103,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   for (int d = 0; d < nd; d++) L.L[d] = I.h.L[d];
   L.t = t;
   L.N = I.h.L[t];
-  int cap = fft_tile_capacity(dbl);
+  int cap = fft_tile_capacity(dbl, 1);
   if (L.N > cap) {
     raise_error(comm, "FFT: axis %d of length %d exceeds the %d-point tile of this build", t, L.N, cap);
     return false;
@@ -130,7 +130,8 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   L.in_row = in_row ? 1 : 0;
   L.out_row = out_row ? 1 : 0;
   if (u >= 0) {
-    L.T = std::max(1, std::min(std::min(cap / L.N, I.h.L[u]), 512));
+    int tcap = fft_tile_capacity(dbl, !(in_row && out_row));
+    L.T = std::max(1, std::min(std::min(tcap / L.N, I.h.L[u]), 512));
     L.tiles_u = (I.h.L[u] + L.T - 1) / L.T;
   } else {
     L.T = 1;
@@ -171,6 +172,46 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     // tile lines follow the address order of the column-walked side
     const Side &cs = out_row ? I : U;
     if (!is_identity(cs.order[u])) pp->ord_u = (long)tb.put(cs.order[u].data(), I.h.L[u] * sizeof(int));
+  }
+  // Per-tile line bases for uniform sides: one coalesced load per tile in
+  // the kernel instead of a chain of lookups (bounded to 64 MiB of tables).
+  {
+    long long nlines = (long long)L.tiles_u * batch * L.T;
+    if (I.h.uniform && U.h.uniform && nlines * 16 <= (64LL << 20)) {
+      std::vector<i64> bi(nlines), bo(nlines);
+      const std::vector<int> *ou = nullptr;
+      if (pp->ord_u >= 0) ou = &((out_row ? I : U).order[u]);
+      std::vector<int> g(nd, 0);
+      for (long long tile = 0; tile < (long long)L.tiles_u * batch; tile++) {
+        int tu = (int)(tile % L.tiles_u);
+        long long bid = tile / L.tiles_u;
+        for (int k = L.nbatch - 1; k >= 0; k--) {
+          int d = L.batch_axis[k];
+          g[d] = (int)(bid % I.h.L[d]);
+          bid /= I.h.L[d];
+        }
+        i64 base_i = 0, base_o = 0;
+        for (int k = 0; k < L.nbatch; k++) {
+          int d = L.batch_axis[k];
+          base_i += I.abs[d][g[d]];
+          base_o += U.abs[d][g[d]];
+        }
+        for (int l = 0; l < L.T; l++) {
+          i64 ai = base_i, ao = base_o;
+          if (u >= 0) {
+            int cu = tu * L.T + l;
+            if (cu >= I.h.L[u]) cu = 0;
+            else if (ou) cu = (*ou)[cu];
+            ai += I.abs[u][cu];
+            ao += U.abs[u][cu];
+          }
+          bi[tile * L.T + l] = ai;
+          bo[tile * L.T + l] = ao;
+        }
+      }
+      pp->line_in = (long)tb.put(bi.data(), bi.size() * sizeof(i64));
+      pp->line_out = (long)tb.put(bo.data(), bo.size() * sizeof(i64));
+    }
   }
   std::vector<double> w = roots(L.N);
   if (dbl) {
@@ -213,6 +254,8 @@ void *finalize_passes(dm_comm_t comm, TableBuilder &tb, std::vector<PendingPass>
     L.ord_in = p.ord_in >= 0 ? (const int *)(b + p.ord_in) : nullptr;
     L.ord_out = p.ord_out >= 0 ? (const int *)(b + p.ord_out) : nullptr;
     L.ord_u = p.ord_u >= 0 ? (const int *)(b + p.ord_u) : nullptr;
+    L.line_in_tab = p.line_in >= 0 ? (const i64 *)(b + p.line_in) : nullptr;
+    L.line_out_tab = p.line_out >= 0 ? (const i64 *)(b + p.line_out) : nullptr;
     L.twiddle = b + p.tw;
     out->push_back(L);
   }

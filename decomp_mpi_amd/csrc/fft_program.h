@@ -40,6 +40,7 @@ struct PendingPass {
   size_t out_off[DM_MAX_DIMS];
   PendingSide sin, sout;
   long ord_in = -1, ord_out = -1, ord_u = -1, tw = -1;
+  long line_in = -1, line_out = -1;
 };
 
 // Plan the pass over axis t from layout I (buffer in) to layout U (buffer

@@ -64,6 +64,8 @@ struct FftLaunch {
   const int *ord_in;          // row mode: t coordinates in input address order (or null)
   const int *ord_out;         // row mode: t coordinates in output address order (or null)
   const int *ord_u;           // tile line l holds u coordinate ord_u[u0 + l] (or u0 + l)
+  const i64 *line_in_tab;     // uniform sides: base offset of line l of tile b at [b * T + l]
+  const i64 *line_out_tab;    //   (null: computed from the per-axis tables)
   const void *twiddle;        // N entries exp(-2 pi i k / N) in the pass precision
   int inverse;                // 0 forward, 1 backward (conjugate roots)
   int dbl;                    // 1: complex double, 0: complex float
@@ -72,6 +74,6 @@ struct FftLaunch {
 };
 void launch_fft_pass(const FftLaunch &p, void *stream);
 // Largest points per tile the FFT pass kernel holds for an element size.
-int fft_tile_capacity(int dbl);
+int fft_tile_capacity(int dbl, int column);
 
 }  // namespace dm

@@ -194,3 +194,23 @@ void launch_reshape(const ReshapeLaunch &p0, void *stream) {
 }
 
 }  // namespace dm
+
+// ---- plain copy, the calibration reference for bandwidth and counters ------
+namespace dm {
+namespace {
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     long long n) {
+  long long i = (long long)blockIdx.x * 256 * 4 + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 4; k++, i += 256)
+    if (i < n) dst[i] = src[i];
+}
+}  // namespace
+
+void launch_copy16(const void *src, void *dst, long long n16, void *stream) {
+  long long blocks = (n16 + 1023) / 1024;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const uint4 *)src, (uint4 *)dst, n16);
+}
+}  // namespace dm

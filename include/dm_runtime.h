@@ -54,6 +54,12 @@ DmAxisFFT dm_plan_axis_fft(struct decomp *in, struct decomp *out, int axis, int 
 int dm_run_axis_fft(DmAxisFFT plan, const void *in, void *out);
 void dm_free_axis_fft(DmAxisFFT plan);
 
+/* ---- calibration -------------------------------------------------------
+ * Average milliseconds of a plain 16-byte-vector device copy of `bytes`
+ * over `reps` launches: the achievable-bandwidth and byte-counter reference
+ * for the roofline figures. */
+double dm_copy_bench(MPI_Comm comm, void *dst, const void *src, size_t bytes, int reps);
+
 /* ---- errors ------------------------------------------------------------
  * mode 0 (default): raise_error prints and aborts, as the reference.
  * mode 1: raise_error records the message and returns; the failing call

@@ -357,8 +357,26 @@ int or_decomp_scale(int P, int ndim, const int *specs, const int *spec_off, cons
 
 /* ---------------------------------------------------------------- stage DFTs */
 
+/* exp(sign 2 pi i k / n) for k < n, computed in long double; the last
+ * table is kept since consecutive calls share n. */
+static const double complex *root_table(int n, int sign) {
+  static double complex *tab = NULL;
+  static int tn = 0, ts = 0;
+  if (tab != NULL && tn == n && ts == sign) return tab;
+  free(tab);
+  tab = malloc(sizeof(double complex) * n);
+  for (int k = 0; k < n; k++) {
+    long double a = (long double)sign * 2.0L * 3.141592653589793238462643383279503L * k / n;
+    tab[k] = (double)cosl(a) + I * (double)sinl(a);
+  }
+  tn = n;
+  ts = sign;
+  return tab;
+}
+
 static void dft1(double complex *x, int n, int sign, double complex *tmp) {
   if (n <= 1) return;
+  const double complex *w = root_table(n, sign);
   if ((n & (n - 1)) == 0) {
     /* iterative radix-2, decimation in time */
     for (int i = 1, j = 0; i < n; i++) {
@@ -373,10 +391,9 @@ static void dft1(double complex *x, int n, int sign, double complex *tmp) {
     }
     for (int len = 2; len <= n; len <<= 1) {
       for (int k = 0; k < len / 2; k++) {
-        long double a = (long double)sign * 2.0L * 3.141592653589793238462643383279503L * k / len;
-        double complex w = (double)cosl(a) + I * (double)sinl(a);
+        double complex wk = w[(long)k * (n / len)];
         for (int i = 0; i < n; i += len) {
-          double complex u = x[i + k], v = x[i + k + len / 2] * w;
+          double complex u = x[i + k], v = x[i + k + len / 2] * wk;
           x[i + k] = u + v;
           x[i + k + len / 2] = u - v;
         }
@@ -387,9 +404,7 @@ static void dft1(double complex *x, int n, int sign, double complex *tmp) {
   for (int k = 0; k < n; k++) {
     double complex s = 0;
     for (int j = 0; j < n; j++) {
-      long m = ((long)j * k) % n;
-      long double a = (long double)sign * 2.0L * 3.141592653589793238462643383279503L * m / n;
-      s += x[j] * ((double)cosl(a) + I * (double)sinl(a));
+      s += x[j] * w[((long)j * k) % n];
     }
     tmp[k] = s;
   }
