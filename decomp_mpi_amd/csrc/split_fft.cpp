@@ -416,6 +416,10 @@ static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, D
 
 static SplitPlan propose(int dbl, int flags, Decomp start, int *initial_factor, void *dat, int length,
                          void *workspace) {
+  if (start == nullptr || initial_factor == nullptr) {
+    raise_error(start ? start->comm : nullptr, "Invalid factor");
+    return nullptr;
+  }
   int ndim = start->ndim;
   int use_first = 1, use_second = 1;
   std::vector<int> A(ndim), B(ndim), L(ndim);

@@ -21,8 +21,8 @@ extern "C" {
 #endif
 
 /** Print "<rank>: Error in reshape:<msg>" on stderr and abort the process
- *  (reference reshape.c:12-21 calls MPI_Abort(comm,555)).  Never returns
- *  unless an error hook is installed (dm_set_error_hook). */
+ *  (reference reshape.c:12-21 calls MPI_Abort(comm,555)).  Returns only
+ *  after dm_set_error_mode(1), which records the message instead. */
 void raise_error(MPI_Comm comm, const char *msg, ...);
 
 typedef struct dim_decomp *DimDecomp;
