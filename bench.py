@@ -52,7 +52,7 @@ def make_input(seed=7):
     return x, modes
 
 
-def setup(C, dm, comm, dbl, host_x):
+def setup(C, dm, comm, dbl, host_x, factors=None):
     dims = [C.makeLocalDecomp(L), C.makeBlockDecomp(L, GRID[1]), C.makeBlockDecomp(L, GRID[2])]
     start = C.makePackedDecomp(comm, 3, dm.ptrs(dims))
     per = C.extent(start)
@@ -61,11 +61,78 @@ def setup(C, dm, comm, dbl, host_x):
     dat = dm.DistBuffer(comm, per * es)
     work = dm.DistBuffer(comm, length * es)
     dat.put_all(host_x if dbl else host_x.astype(np.complex64))
-    tune = C.autotuneParallelPlan if dbl else C.autotuneParallelPlanf
-    plan = tune(start, dat.ptr, length, work.ptr)
+    if factors is None:
+        tune = C.autotuneParallelPlan if dbl else C.autotuneParallelPlanf
+        plan = tune(start, dat.ptr, length, work.ptr)
+    else:
+        make = C.makeParallelPlan if dbl else C.makeParallelPlanf
+        plan = make(start, dm.ints(factors), dat.ptr, length, work.ptr)
     if not plan:
         raise RuntimeError("autotune produced no plan")
     return start, dat, work, plan, per
+
+
+def reshape_chain(C, dm, comm, reps=10):
+    """The 512^3 int64 reshape chain on the same 4x8 grid: block ->
+    rotated (seeded shifts, memory order z slowest, x, y fastest) ->
+    interleave (stride 4 on axis 0).  Returns ms per hop and GB/s at the
+    algorithmic 2 * N * 8 bytes per hop."""
+    shifts = [int(v) for v in np.random.default_rng(11).integers(0, L, 3)]
+    block = C.makeDecomp(3, dm.ptrs([C.makeAxisDecomp(comm, C.makeBlockDecomp(L, 4), 64 * L, 8),
+                                     C.makeAxisDecomp(comm, C.makeBlockDecomp(L, 8), L, 1),
+                                     C.makeAxisDecomp(comm, C.makeLocalDecomp(L), 1, 1)]))
+    rot = []
+    for sh, nested in zip(shifts, [C.makeBlockDecomp(L, 4), C.makeBlockDecomp(L, 8), C.makeLocalDecomp(L)]):
+        rot.append(C.makeRotatedDecomp(sh, nested))
+        C.freeDimDecomp(nested)
+    rotated = C.makeDecomp(3, dm.ptrs([C.makeAxisDecomp(comm, rot[0], 64, 8), C.makeAxisDecomp(comm, rot[1], 1, 1),
+                                       C.makeAxisDecomp(comm, rot[2], 128 * 64, 1)]))
+    inner = C.makeBlockDecomp(4, 4)
+    il = C.makeInterleaveDecomp(inner, 128)
+    C.freeDimDecomp(inner)
+    inter = C.makePackedDecomp(comm, 3, dm.ptrs([il, C.makeBlockDecomp(L, 8), C.makeLocalDecomp(L)]))
+    t = dm.mpi_type("long_long")
+    plans = [C.makeReshapePlan(comm, t, block, rotated), C.makeReshapePlan(comm, t, rotated, inter)]
+    slots = [C.extent(d) for d in (block, rotated, inter)]
+    bufs = [dm.DistBuffer(comm, s_ * 8) for s_ in slots]
+    for b, s_ in zip(bufs, slots):
+        b.fill(0, s_ * 8)
+    C.runReshapePlan(comm, plans[0], bufs[0].ptr, bufs[1].ptr)
+    C.runReshapePlan(comm, plans[1], bufs[1].ptr, bufs[2].ptr)
+    hop_ms = []
+    for k, (a, b) in enumerate(((0, 1), (1, 2))):
+        C.dm_synchronize(comm)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            C.runReshapePlan(comm, plans[k], bufs[a].ptr, bufs[b].ptr)
+        C.dm_synchronize(comm)
+        hop_ms.append((time.perf_counter() - t0) / reps * 1e3)
+    for p_ in plans:
+        C.freeReshapePlan(p_)
+    for b in bufs:
+        b.free()
+    for d in (block, rotated, inter):
+        C.freeDecomp(d)
+    alg = 2 * L ** 3 * 8
+    gbs = [alg / (ms * 1e-3) / 1e9 for ms in hop_ms]
+    return {"workload": "512^3 int64 reshape chain block -> rotate(z,x,y) -> interleave(4) on 4x8 virtual ranks",
+            "shifts": shifts, "ms_per_hop": [round(v, 4) for v in hop_ms],
+            "GB_per_s": [round(v, 1) for v in gbs],
+            "frac_of_peak": [round(v / PEAK_HBM_GBS, 4) for v in gbs]}
+
+
+def measured_traffic(bytes_per_launch):
+    """HBM bytes per FFT pass launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what it
+    reads for a plain copy of known size), when that summary exists."""
+    path = os.path.join(ROOT, "profiles", "fft_pass_512cube_c128_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bytes": int(t["hbm_bytes_per_launch"]),
+            "ratio_to_algorithmic": round(t["hbm_bytes_per_launch"] / bytes_per_launch, 4),
+            "source": "profiles/fft_pass_512cube_c128_traffic.json"}
 
 
 def time_steps(C, dm, comm, plan, steps, warmup):
@@ -86,9 +153,11 @@ def time_steps(C, dm, comm, plan, steps, warmup):
     return wall, kernel_s
 
 
-def cpu_baseline(n=256):
-    """The oracle (C restatement of the reference path, one core) on a
-    bounded sample: the same pipeline on an n^3 cube over the same 4x8 grid."""
+def cpu_baseline(n=512):
+    """The oracle (C restatement of the reference path, one core): one
+    forward transform of the same 512^3 pipeline on the same 4x8 grid with
+    the reference's default split (A = 16, B = 32), about 20 s of CPU work;
+    GB/s at the same algorithmic 6 * N * 16 bytes per transform."""
     import oracle
     start = oracle.Decomp(P, [(oracle.Local(n), 32), (oracle.Block(n, 4), 8), (oracle.Block(n, 8), 1)])
     a = 1
@@ -105,11 +174,10 @@ def cpu_baseline(n=256):
     w = np.zeros((P, 2 * ext), np.complex128)
     t0 = time.perf_counter()
     oracle.split_fft(start, A, B, ga, gb, x, w, ext, -1)
-    oracle.split_fft(start, A, B, ga, gb, x, w, ext, +1)
     dt = time.perf_counter() - t0
-    alg = 2 * 6 * n ** 3 * 16
+    alg = 6 * n ** 3 * 16
     return {"value": round(alg / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle split FFT forward+inverse, {n}^3 complex double, 4x8 virtual grid, "
+            "sample": f"oracle split FFT, one forward transform, {n}^3 complex double, 4x8 virtual grid, "
                       f"A={A}, {dt:.2f} s on one host core"}
 
 
@@ -120,6 +188,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-reshape", action="store_true")
+    ap.add_argument("--factors", type=str, default=None,
+                    help="B factors b0,b1,b2 for makeParallelPlan instead of the autotuner (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the virtual-rank grid stands in for "
@@ -136,7 +207,8 @@ def main():
     alg_step = 2 * 6 * N * 16
 
     print("bench: planning FP64", file=sys.stderr, flush=True)
-    start, dat, work, plan, per = setup(C, dm, comm, True, host_x)
+    factors = [int(v) for v in args.factors.split(",")] if args.factors else None
+    start, dat, work, plan, per = setup(C, dm, comm, True, host_x, factors)
     sp = dm.split_plan(plan)
     A64 = [sp.A_factors[i] for i in range(3)]
     # correctness spot checks on the timed path: planted peaks and round trip
@@ -164,7 +236,7 @@ def main():
     fp32 = None
     if not args.no_fp32:
         print("bench: planning + timing FP32", file=sys.stderr, flush=True)
-        start32, dat32, work32, plan32, _ = setup(C, dm, comm, False, host_x)
+        start32, dat32, work32, plan32, _ = setup(C, dm, comm, False, host_x, factors)
         sp32 = dm.split_plan(plan32)
         A32 = [sp32.A_factors[i] for i in range(3)]
         wall32, kern32 = time_steps(C, dm, comm, plan32, args.steps, args.warmup)
@@ -194,6 +266,11 @@ def main():
     copy_gbs = 2 * half / (copy_ms * 1e-3) / 1e9
     cbuf.free()
 
+    chain = None
+    if not args.no_reshape:
+        print("bench: timing the 512^3 int64 reshape chain", file=sys.stderr, flush=True)
+        chain = reshape_chain(C, dm, comm)
+
     cpu = None if args.no_cpu else cpu_baseline()
     value = alg_step / (wall / args.steps) / 1e9
     out = {
@@ -212,9 +289,11 @@ def main():
         "config": {"workload": "split FFT forward+inverse, 512^3 complex double, pencil start "
                                "{Local(512), Block(512,4) rs8, Block(512,8) rs1} on 32 virtual ranks",
                    "A_factors": A64, "gflops": round(2 * 5 * N * np.log2(N) / (wall / args.steps) / 1e9, 1),
-                   "round_trip_rel_l2": rt64, "fp32_form": fp32},
+                   "round_trip_rel_l2": rt64, "fp32_form": fp32, "reshape_chain": chain},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": (measured_traffic(pass_bytes) or {}).get("bytes"),
+                     "traffic_detail": measured_traffic(pass_bytes),
                      "kernel": "fft_pass_kernel<double> (one axis pass)",
                      "pass_ms": round(pass_ms, 4), "bytes_per_launch": pass_bytes,
                      "copy_2GiB_GBps": round(copy_gbs, 1)},
