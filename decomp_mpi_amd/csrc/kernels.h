@@ -21,6 +21,7 @@ struct ReshapeLaunch {
   int nbatch;
   int batch_axis[DM_MAX_DIMS];
   int tiles_a, tiles_b;
+  long long nbatch_total;      // product of the batch axes' lengths
   long long nblocks;
   const char *src_base;   // slot 0 pointer (already offset by the caller's pointer)
   char *dst_base;

@@ -325,6 +325,13 @@ bool enqueue_reshape(dm_comm_t comm, ReshapePlan plan, const void *src, void *ds
       return false;
     }
   }
+  // Nothing owned on either side: nothing to move.  (The kernels read
+  // element 0 of the source region in place of absent elements, which is in
+  // bounds whenever some rank's source extent is positive.)
+  size_t src_max = 0, dst_max = 0;
+  for (auto e : pg->src.slot_extent) src_max = std::max(src_max, (size_t)e);
+  for (auto e : pg->dst.slot_extent) dst_max = std::max(dst_max, (size_t)e);
+  if (src_max == 0 || dst_max == 0) return true;
   if (!pg->uploaded) {
     pg->dsrc.upload(pg->src);
     pg->ddst.upload(pg->dst);

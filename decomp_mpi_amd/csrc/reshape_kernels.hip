@@ -18,6 +18,7 @@ namespace {
 constexpr int TILE = 32;      // tile edge (elements)
 constexpr int ROWS = 8;       // thread rows per tile pass
 constexpr int NTHREADS = TILE * ROWS;
+constexpr int TILE_SLICES = 4;   // batch slices per 2D-tile workgroup
 
 template <int BYTES>
 struct Elem;
@@ -34,17 +35,21 @@ struct Partial {
   int off[DM_MAX_DIMS];       // non-uniform: the batch axes' local offsets
 };
 
-__device__ __forceinline__ void partial_init(const DevLayout &Lt, const ReshapeLaunch &p,
-                                             const int *coord, Partial &q) {
+// Batch part of a workgroup's coordinates: batch index bidx decoded over
+// the batch axes (last batch axis fastest) straight into the sums, so no
+// coordinate array is indexed at run time.
+template <bool UNIFORM>
+__device__ __forceinline__ void partial_init(const DevLayout &Lt, const ReshapeLaunch &p, long long bidx,
+                                             Partial &q) {
   q.rank = 0;
   q.addr = 0;
-  for (int k = 0; k < p.nbatch; k++) {
+  for (int k = p.nbatch - 1; k >= 0; k--) {
     int d = p.batch_axis[k];
-    int g = coord[d];
+    int g = (int)(bidx % p.L[d]);
+    bidx /= p.L[d];
     q.rank += Lt.rank[d][g];
-    int o = Lt.off[d][g];
-    q.off[k] = o;
-    if (Lt.uniform) q.addr += Lt.addr[d][g];
+    if (UNIFORM) q.addr += Lt.addr[d][g];
+    else q.off[k] = Lt.off[d][g];
   }
 }
 
@@ -68,62 +73,85 @@ __device__ __forceinline__ i64 locate(const DevLayout &Lt, const ReshapeLaunch &
   return (i64)R * slot + off;
 }
 
-__device__ __forceinline__ void decode_batch(const ReshapeLaunch &p, long long bidx, int *coord) {
-  for (int k = p.nbatch - 1; k >= 0; k--) {
-    int d = p.batch_axis[k];
-    coord[d] = (int)(bidx % p.L[d]);
-    bidx /= p.L[d];
-  }
-}
-
 // 2D tile: read rows along b (source-contiguous), write rows along a.
-template <int BYTES, bool US, bool UD>
+// A workgroup moves NB tiles that differ only in the batch coordinate (so
+// the tile-axis tables are shared and more bytes are in flight per
+// workgroup).  Each side runs in three branch-free phases: every address of
+// the thread's elements (table loads only), then every data load (clamped
+// to element 0 of the region when the element is absent), then the LDS (or
+// global) writes.  A load queue that completes in order would otherwise
+// stall each data load behind the next element's table loads.
+template <int BYTES, bool US, bool UD, int NB>
 __global__ __launch_bounds__(NTHREADS) void reshape_tile2d(ReshapeLaunch p) {
   typedef typename Elem<BYTES>::T E;
-  __shared__ E tile[TILE][TILE + 1];
-  __shared__ unsigned char have[TILE][TILE + 1];
+  constexpr int RPT = TILE / ROWS;
+  __shared__ E tile[NB][TILE][TILE + 1];
+  __shared__ unsigned char have[NB][TILE][TILE + 1];
 
   long long bid = blockIdx.x;
   int ta = (int)(bid % p.tiles_a);
   bid /= p.tiles_a;
   int tb = (int)(bid % p.tiles_b);
   bid /= p.tiles_b;
-  int coord[DM_MAX_DIMS];
-  decode_batch(p, bid, coord);
-  Partial qs, qd;
-  partial_init(p.src, p, coord, qs);
-  partial_init(p.dst, p, coord, qd);
+  Partial qs[NB], qd[NB];
+  bool live[NB];
+#pragma unroll
+  for (int n = 0; n < NB; n++) {
+    long long b = bid * NB + n;
+    live[n] = b < p.nbatch_total;
+    partial_init<US>(p.src, p, live[n] ? b : 0, qs[n]);
+    partial_init<UD>(p.dst, p, live[n] ? b : 0, qd[n]);
+  }
 
   const E *src = reinterpret_cast<const E *>(p.src_base);
   E *dst = reinterpret_cast<E *>(p.dst_base);
+  const int La = p.L[p.a], Lb = p.L[p.b];
   int a0 = ta * TILE, b0 = tb * TILE;
   int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
 
   // load: consecutive threads walk axis b
-  for (int i = ty; i < TILE; i += ROWS) {
-    int ga = a0 + i, gb = b0 + tx;
-    unsigned char ok = 0;
-    if (ga < p.L[p.a] && gb < p.L[p.b]) {
-      i64 s = locate<US>(p.src, p, qs, p.a, ga, p.b, gb, p.src_slot);
-      if (s >= 0) {
-        tile[i][tx] = src[s];
-        ok = 1;
-      }
+  i64 s[NB][RPT];
+#pragma unroll
+  for (int n = 0; n < NB; n++)
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      int ga = a0 + ty + k * ROWS, gb = b0 + tx;
+      bool in = live[n] && ga < La && gb < Lb;
+      s[n][k] = locate<US>(p.src, p, qs[n], p.a, in ? ga : 0, p.b, in ? gb : 0, p.src_slot);
+      if (!in) s[n][k] = -1;
     }
-    have[i][tx] = ok;
-  }
+  E v[NB][RPT];
+#pragma unroll
+  for (int n = 0; n < NB; n++)
+#pragma unroll
+    for (int k = 0; k < RPT; k++) v[n][k] = src[s[n][k] >= 0 ? s[n][k] : 0];
+#pragma unroll
+  for (int n = 0; n < NB; n++)
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      tile[n][ty + k * ROWS][tx] = v[n][k];
+      have[n][ty + k * ROWS][tx] = s[n][k] >= 0;
+    }
   __syncthreads();
   // store: consecutive threads walk axis a
-  for (int j = ty; j < TILE; j += ROWS) {
-    int ga = a0 + tx, gb = b0 + j;
-    if (ga < p.L[p.a] && gb < p.L[p.b] && have[tx][j]) {
-      i64 d = locate<UD>(p.dst, p, qd, p.a, ga, p.b, gb, p.dst_slot);
-      if (d >= 0) dst[d] = tile[tx][j];
+  i64 d[NB][RPT];
+#pragma unroll
+  for (int n = 0; n < NB; n++)
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      int ga = a0 + tx, gb = b0 + ty + k * ROWS;
+      bool in = live[n] && ga < La && gb < Lb;
+      d[n][k] = locate<UD>(p.dst, p, qd[n], p.a, in ? ga : 0, p.b, in ? gb : 0, p.dst_slot);
+      if (!in || !have[n][tx][ty + k * ROWS]) d[n][k] = -1;
     }
-  }
+#pragma unroll
+  for (int n = 0; n < NB; n++)
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+      if (d[n][k] >= 0) dst[d[n][k]] = tile[n][tx][ty + k * ROWS];
 }
 
-// 1D runs along axis a (same fastest axis on both sides).
+// 1D runs along axis a (same fastest axis on both sides), phased as above.
 template <int BYTES, bool US, bool UD>
 __global__ __launch_bounds__(NTHREADS) void reshape_run1d(ReshapeLaunch p) {
   typedef typename Elem<BYTES>::T E;
@@ -131,29 +159,37 @@ __global__ __launch_bounds__(NTHREADS) void reshape_run1d(ReshapeLaunch p) {
   long long bid = blockIdx.x;
   int ta = (int)(bid % p.tiles_a);
   bid /= p.tiles_a;
-  int coord[DM_MAX_DIMS];
-  decode_batch(p, bid, coord);
   Partial qs, qd;
-  partial_init(p.src, p, coord, qs);
-  partial_init(p.dst, p, coord, qd);
+  partial_init<US>(p.src, p, bid, qs);
+  partial_init<UD>(p.dst, p, bid, qd);
   const E *src = reinterpret_cast<const E *>(p.src_base);
   E *dst = reinterpret_cast<E *>(p.dst_base);
+  const int La = p.L[p.a];
   int base = ta * NTHREADS * PER;
+  i64 s[PER], d[PER];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     int ga = base + k * NTHREADS + threadIdx.x;
-    if (ga >= p.L[p.a]) break;
-    i64 s = locate<US>(p.src, p, qs, p.a, ga, p.a, ga, p.src_slot);
-    i64 d = locate<UD>(p.dst, p, qd, p.a, ga, p.a, ga, p.dst_slot);
-    if (s >= 0 && d >= 0) dst[d] = src[s];
+    bool in = ga < La;
+    int g = in ? ga : 0;
+    s[k] = locate<US>(p.src, p, qs, p.a, g, p.a, g, p.src_slot);
+    d[k] = locate<UD>(p.dst, p, qd, p.a, g, p.a, g, p.dst_slot);
+    if (!in || s[k] < 0 || d[k] < 0) s[k] = d[k] = -1;
   }
+  E v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) v[k] = src[s[k] >= 0 ? s[k] : 0];
+#pragma unroll
+  for (int k = 0; k < PER; k++)
+    if (d[k] >= 0) dst[d[k]] = v[k];
 }
 
 template <int BYTES, bool US, bool UD>
 void launch_typed(const ReshapeLaunch &p, hipStream_t s) {
   if (p.nblocks <= 0) return;
   if (p.a != p.b)
-    hipLaunchKernelGGL((reshape_tile2d<BYTES, US, UD>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0, s, p);
+    hipLaunchKernelGGL((reshape_tile2d<BYTES, US, UD, TILE_SLICES>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0,
+                       s, p);
   else
     hipLaunchKernelGGL((reshape_run1d<BYTES, US, UD>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0, s, p);
 }
@@ -175,14 +211,16 @@ void launch_reshape(const ReshapeLaunch &p0, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   long long batch = 1;
   for (int k = 0; k < p.nbatch; k++) batch *= p.L[p.batch_axis[k]];
+  p.nbatch_total = batch;
   if (p.a != p.b) {
     p.tiles_a = (p.L[p.a] + TILE - 1) / TILE;
     p.tiles_b = (p.L[p.b] + TILE - 1) / TILE;
+    p.nblocks = (long long)p.tiles_a * p.tiles_b * ((batch + TILE_SLICES - 1) / TILE_SLICES);
   } else {
     p.tiles_a = (p.L[p.a] + NTHREADS * 4 - 1) / (NTHREADS * 4);
     p.tiles_b = 1;
+    p.nblocks = (long long)p.tiles_a * batch;
   }
-  p.nblocks = (long long)p.tiles_a * p.tiles_b * batch;
   switch (p.elem_bytes) {
     case 1: launch_bytes<1>(p, s); break;
     case 2: launch_bytes<2>(p, s); break;
