@@ -205,7 +205,7 @@ __device__ __forceinline__ int fdiv(int n, unsigned long long magic) {
 struct StageInfo {
   unsigned long long m_nr;   // magic for N / radix
   unsigned long long m_ns;   // magic for Ns
-  int nr, ns, twstride;
+  int nr, ns;
 };
 
 struct Pass {
@@ -252,9 +252,11 @@ __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx
         v[k][r] = buf[base + lds_pad(x)];
       }
       if (ns > 1) {
+        // stage-ordered table: consecutive jj read consecutive entries
+        const cx<R> *tws = tw + (ns - 1) + jj;
 #pragma unroll
         for (int r = 1; r < RADIX; r++) {
-          cx<R> w = tw[jj * r * si.twstride];
+          cx<R> w = tws[(r - 1) * ns];
           if (INV) w.y = -w.y;
           v[k][r] = cmul(v[k][r], w);
         }
@@ -464,10 +466,10 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
     int base = l * P.ls;
 #pragma unroll
     for (int r = 0; r < RADIX; r++) v[q][r] = buf[base + lds_pad(j + r * nr)];
-    // in the last stage ns = nr, so j % ns = j and the twiddle stride is 1
+    // in the last stage ns = nr, so j % ns = j (stage-ordered table)
 #pragma unroll
     for (int r = 1; r < RADIX; r++) {
-      cx<R> w = tw[j * r];
+      cx<R> w = tw[(nr - 1) + (r - 1) * nr + j];
       if (INV) w.y = -w.y;
       v[q][r] = cmul(v[q][r], w);
     }
@@ -713,7 +715,7 @@ size_t fft_lds_bytes(const FftLaunch &p) {
 }
 
 void launch_fft_pass(const FftLaunch &p, void *stream) {
-  Pass P;
+  Pass P{};
   P.p = p;
   P.m_n = magic_for(p.N);
   P.m_t = magic_for(p.T);
@@ -745,7 +747,6 @@ void launch_fft_pass(const FftLaunch &p, void *stream) {
     P.st[s].ns = ns;
     P.st[s].m_nr = magic_for(p.N / r);
     P.st[s].m_ns = magic_for(ns);
-    P.st[s].twstride = p.N / (ns * r);
     ns *= r;
   }
   long long batch = 1;

@@ -213,7 +213,27 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
       pp->line_out = (long)tb.put(bo.data(), bo.size() * sizeof(i64));
     }
   }
-  std::vector<double> w = roots(L.N);
+  // Twiddles in stage order: stage s (Ns = product of the earlier radices,
+  // radix R) holds w^(jj * r * N / (Ns * R)) for r = 1..R-1, jj < Ns at
+  // index (Ns - 1) + (r - 1) * Ns + jj, so the lanes of a wave (consecutive
+  // jj) read consecutive entries.  The stages telescope to N - 1 entries.
+  std::vector<double> root = roots(L.N);
+  std::vector<double> w(2 * (size_t)L.N, 0.0);
+  {
+    int ns = 1;
+    for (int st = 0; st < L.nstage; st++) {
+      int R = L.radix[st];
+      int stride = L.N / (ns * R);
+      for (int r = 1; r < R; r++)
+        for (int jj = 0; jj < ns; jj++) {
+          size_t at = (size_t)(ns - 1) + (size_t)(r - 1) * ns + jj;
+          size_t k = (size_t)jj * r * stride;
+          w[2 * at] = root[2 * k];
+          w[2 * at + 1] = root[2 * k + 1];
+        }
+      ns *= R;
+    }
+  }
   if (dbl) {
     pp->tw = (long)tb.put(w.data(), w.size() * sizeof(double));
   } else {

@@ -67,7 +67,9 @@ struct FftLaunch {
   const int *ord_u;           // tile line l holds u coordinate ord_u[u0 + l] (or u0 + l)
   const i64 *line_in_tab;     // uniform sides: base offset of line l of tile b at [b * T + l]
   const i64 *line_out_tab;    //   (null: computed from the per-axis tables)
-  const void *twiddle;        // N entries exp(-2 pi i k / N) in the pass precision
+  const void *twiddle;        // stage-ordered roots (N entries, pass precision): stage with
+                              //   Ns earlier points and radix R holds w^(jj*r*N/(Ns*R)) at
+                              //   (Ns-1) + (r-1)*Ns + jj
   int inverse;                // 0 forward, 1 backward (conjugate roots)
   int dbl;                    // 1: complex double, 0: complex float
   const void *in;
