@@ -128,6 +128,12 @@ _SIGS = {
     "gcd": (ctypes.c_uint, [ctypes.c_uint, ctypes.c_uint]), "decomp_scale": (None, [P, P, P, I]),
     "makeProcessorGrid": (IP, [P, I, IP]), "visitData": (None, [P, P, P]),
     "timer": (D, []),
+    # batched local transforms (dm_batch.h)
+    "dm_batch_plan": (P, [I, I, IP, I, I]), "dm_batch_bind": (I, [P, ctypes.POINTER(P)]),
+    "dm_batch_set_schedule": (I, [P, I, I]), "dm_batch_run": (I, [P, P]),
+    "dm_batch_graph_launch": (I, [P, P]), "dm_batch_autotune": (D, [P, P]),
+    "dm_batch_info": (I, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), IP, IP]),
+    "dm_batch_bytes": (D, [P]), "dm_batch_free": (None, [P]),
 }
 
 # predefined element types (addresses of exported structs)

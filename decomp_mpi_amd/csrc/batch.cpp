@@ -1,0 +1,382 @@
+// Batched local FFTs (dm_batch.h): the host planner for K3.
+//
+// Every transform is a C-order array (last axis contiguous).  The pass over
+// axis d of a transform of shape n has lines of N = n[d] points at point
+// stride inner = prod(n[e], e > d), indexed by (outer, inner) coordinates:
+//   * inner == 1 (the contiguous axis): a tile is T consecutive lines, each
+//     a contiguous run of N points (row walk);
+//   * otherwise a tile is T consecutive inner coordinates of one outer
+//     coordinate, so a row of the tile is T contiguous elements (column
+//     walk).
+// One launch per axis covers the tiles of all transforms (T and the radix
+// schedule are per line length).  A schedule's twiddles are stage ordered
+// like the distributed FFT's: stage with Ns earlier points and radix R holds
+// w_N^(jj r N / (Ns R)) at (Ns - 1) + (r - 1) Ns + jj.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "dm_batch.h"
+#include "fft_program.h"
+#include "internal.h"
+#include "kernels.h"
+
+namespace {
+
+unsigned long long magic(int d) { return (0x100000000ULL / (unsigned long long)d) + 1ULL; }
+
+struct PassTables {
+  std::vector<dm::BatchTile> tiles;
+  void *dev = nullptr;
+  long long grid = 0;
+};
+
+}  // namespace
+
+struct dm_batch {
+  int count = 0, ndim = 0, dbl = 1, inverse = 0;
+  std::vector<int> shapes;
+  int tile_choice = DM_BATCH_TILE_FULL, radix_order = DM_BATCH_RADIX_LARGE_FIRST;
+  // bound state
+  std::vector<void *> data;
+  void *d_ptrs = nullptr;
+  void *d_sched = nullptr;
+  void *d_tw = nullptr;
+  size_t lds = 0;
+  int primes = 0;  // some schedule has a radix 5, 7, 11 or 13
+  std::vector<PassTables> passes;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void *graph_stream = nullptr;
+};
+
+static void drop_graph(DmBatch b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_stream = nullptr;
+}
+
+static void release_bound(DmBatch b) {
+  drop_graph(b);
+  for (auto &p : b->passes) dm::device_free(p.dev);
+  b->passes.clear();
+  dm::device_free(b->d_ptrs);
+  dm::device_free(b->d_sched);
+  dm::device_free(b->d_tw);
+  b->d_ptrs = b->d_sched = b->d_tw = nullptr;
+}
+
+extern "C" DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int precision, int direction) {
+  if (count < 1 || ndim < 1 || ndim > 3 || shapes == nullptr) {
+    raise_error(nullptr, "dm_batch_plan: need count >= 1, 1 <= ndim <= 3 and shapes");
+    return nullptr;
+  }
+  if (direction != -1 && direction != 1) {
+    raise_error(nullptr, "dm_batch_plan: direction must be FFTW_FORWARD (-1) or FFTW_BACKWARD (+1)");
+    return nullptr;
+  }
+  std::vector<int> rad;
+  const int cap = dm::batch_tile_capacity(precision ? 1 : 0);
+  for (long long i = 0; i < (long long)count * ndim; i++) {
+    int n = shapes[i];
+    if (n < 1 || !dm::choose_radices(n, &rad) || rad.size() > 16 || n > cap) {
+      raise_error(nullptr, "dm_batch_plan: axis length %d of transform %lld is not supported", n, i / ndim);
+      return nullptr;
+    }
+  }
+  DmBatch b = new dm_batch;
+  b->count = count;
+  b->ndim = ndim;
+  b->dbl = precision ? 1 : 0;
+  b->inverse = direction == 1 ? 1 : 0;
+  b->shapes.assign(shapes, shapes + (size_t)count * ndim);
+  return b;
+}
+
+static bool bind_tables(DmBatch b) {
+  release_bound(b);
+  const int cap = dm::batch_tile_capacity(b->dbl);
+  // schedules per (N, T): radix list, magics, twiddle block
+  std::map<std::pair<int, int>, int> sched_of;
+  std::vector<dm::BatchSched> sched;
+  std::vector<double> tw;  // complex doubles, converted for FP32 below
+  auto schedule = [&](int N, int T) -> int {
+    auto key = std::make_pair(N, T);
+    auto it = sched_of.find(key);
+    if (it != sched_of.end()) return it->second;
+    dm::BatchSched S{};
+    std::vector<int> rad;
+    dm::choose_radices(N, &rad);
+    if (b->radix_order == DM_BATCH_RADIX_SMALL_FIRST) std::reverse(rad.begin(), rad.end());
+    S.N = N;
+    S.nstage = (int)rad.size();
+    S.T = T;
+    S.m_n = magic(N);
+    S.m_t = magic(T);
+    int ns = 1;
+    S.tw_off = (int)(tw.size() / 2);
+    std::vector<double> root = dm::roots(N);
+    std::vector<double> block(2 * (size_t)N, 0.0);
+    for (int s = 0; s < S.nstage; s++) {
+      int R = rad[s];
+      S.radix[s] = R;
+      if (R == 5 || R == 7 || R == 11 || R == 13) b->primes = 1;
+      S.st[s].nr = N / R;
+      S.st[s].ns = ns;
+      S.st[s].m_nr = magic(N / R);
+      S.st[s].m_ns = magic(ns);
+      int stride = N / (ns * R);
+      for (int r = 1; r < R; r++)
+        for (int jj = 0; jj < ns; jj++) {
+          size_t at = (size_t)(ns - 1) + (size_t)(r - 1) * ns + jj;
+          size_t k = (size_t)jj * r * stride;
+          block[2 * at] = root[2 * k];
+          block[2 * at + 1] = root[2 * k + 1];
+        }
+      ns *= R;
+    }
+    tw.insert(tw.end(), block.begin(), block.end());
+    int id = (int)sched.size();
+    sched.push_back(S);
+    sched_of[key] = id;
+    b->lds = std::max(b->lds, dm::batch_lds_bytes(b->dbl, T, N));
+    return id;
+  };
+  b->lds = 0;
+  b->primes = 0;
+  b->passes.assign(b->ndim, PassTables());
+  for (int i = 0; i < b->count; i++) {
+    const int *n = &b->shapes[(size_t)i * b->ndim];
+    for (int d = 0; d < b->ndim; d++) {
+      long long inner = 1, outer = 1;
+      for (int e = d + 1; e < b->ndim; e++) inner *= n[e];
+      for (int e = 0; e < d; e++) outer *= n[e];
+      const int N = n[d];
+      const bool row = inner == 1;
+      // lines per tile: as many as the tile holds; a column walk never more
+      // than the contiguous run (so no tile is mostly padding)
+      const int limit = b->tile_choice == DM_BATCH_TILE_HALF ? cap / 2 : cap;
+      int T = std::max(1, limit / N);
+      if (!row) T = (int)std::min<long long>(T, inner);
+      int sid = schedule(N, T);
+      auto &tl = b->passes[b->ndim - 1 - d].tiles;  // passes run last axis first
+      if (row) {
+        for (long long o = 0; o < outer; o += T) {
+          dm::BatchTile t{};
+          t.base = o * N;
+          t.ps = 1;
+          t.xform = i;
+          t.sched = sid;
+          t.nl = (int)std::min<long long>(T, outer - o);
+          t.row = 1;
+          tl.push_back(t);
+        }
+      } else {
+        for (long long o = 0; o < outer; o++)
+          for (long long c = 0; c < inner; c += T) {
+            dm::BatchTile t{};
+            t.base = o * N * inner + c;
+            t.ps = inner;
+            t.xform = i;
+            t.sched = sid;
+            t.nl = (int)std::min<long long>(T, inner - c);
+            t.row = 0;
+            tl.push_back(t);
+          }
+      }
+    }
+  }
+  // upload
+  auto up = [&](const void *src, size_t bytes) -> void * {
+    void *d = dm::device_alloc(std::max<size_t>(bytes, 16));
+    if (d == nullptr) return nullptr;
+    if (hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      dm::device_free(d);
+      return nullptr;
+    }
+    return d;
+  };
+  b->d_ptrs = up(b->data.data(), b->data.size() * sizeof(void *));
+  b->d_sched = up(sched.data(), sched.size() * sizeof(dm::BatchSched));
+  if (b->dbl) {
+    b->d_tw = up(tw.data(), tw.size() * sizeof(double));
+  } else {
+    std::vector<float> twf(tw.begin(), tw.end());
+    b->d_tw = up(twf.data(), twf.size() * sizeof(float));
+  }
+  if (!b->d_ptrs || !b->d_sched || !b->d_tw) {
+    raise_error(nullptr, "dm_batch_bind: device allocation or copy failed");
+    return false;
+  }
+  long long resident = dm::batch_resident_groups(b->dbl, b->primes, b->lds);
+  for (auto &p : b->passes) {
+    p.dev = up(p.tiles.data(), p.tiles.size() * sizeof(dm::BatchTile));
+    if (!p.dev) {
+      raise_error(nullptr, "dm_batch_bind: device allocation or copy failed");
+      return false;
+    }
+    p.grid = std::min<long long>((long long)p.tiles.size(), resident);
+  }
+  return true;
+}
+
+extern "C" int dm_batch_bind(DmBatch b, void *const *data) {
+  if (b == nullptr || data == nullptr) {
+    raise_error(nullptr, "dm_batch_bind: NULL batch or data");
+    return 1;
+  }
+  if (!dm::device_available()) {
+    raise_error(nullptr, "dm_batch_bind: no HIP device is available");
+    return 1;
+  }
+  for (int i = 0; i < b->count; i++) {
+    if (data[i] == nullptr) {
+      raise_error(nullptr, "dm_batch_bind: NULL data pointer for transform %d", i);
+      return 1;
+    }
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, data[i]) != hipSuccess || at.type != hipMemoryTypeDevice) {
+      (void)hipGetLastError();
+      raise_error(nullptr, "dm_batch_bind: data of transform %d is not device memory", i);
+      return 1;
+    }
+  }
+  b->data.assign(data, data + b->count);
+  return bind_tables(b) ? 0 : 1;
+}
+
+extern "C" int dm_batch_set_schedule(DmBatch b, int tile, int radix_order) {
+  if (b == nullptr) return 1;
+  b->tile_choice = tile == DM_BATCH_TILE_HALF ? DM_BATCH_TILE_HALF : DM_BATCH_TILE_FULL;
+  b->radix_order = radix_order == DM_BATCH_RADIX_SMALL_FIRST ? DM_BATCH_RADIX_SMALL_FIRST
+                                                              : DM_BATCH_RADIX_LARGE_FIRST;
+  if (b->data.empty()) return 0;
+  return bind_tables(b) ? 0 : 1;
+}
+
+extern "C" int dm_batch_run(DmBatch b, void *stream) {
+  if (b == nullptr || b->passes.empty()) {
+    raise_error(nullptr, "dm_batch_run: batch not bound (dm_batch_bind)");
+    return 1;
+  }
+  for (auto &p : b->passes) {
+    dm::BatchLaunch L{};
+    L.tiles = (const dm::BatchTile *)p.dev;
+    L.ntiles = (long long)p.tiles.size();
+    L.sched = (const dm::BatchSched *)b->d_sched;
+    L.data = (void *const *)b->d_ptrs;
+    L.twiddle = b->d_tw;
+    L.dbl = b->dbl;
+    L.inverse = b->inverse;
+    L.primes = b->primes;
+    L.grid = p.grid;
+    L.lds = b->lds;
+    dm::launch_batch_pass(L, stream);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+extern "C" int dm_batch_graph_launch(DmBatch b, void *stream) {
+  if (b == nullptr || b->passes.empty()) {
+    raise_error(nullptr, "dm_batch_graph_launch: batch not bound (dm_batch_bind)");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (b->exec == nullptr || b->graph_stream != stream) {
+    drop_graph(b);
+    hipStream_t cap = s;
+    bool own = false;
+    if (cap == nullptr) {  // capture needs a non-legacy stream
+      if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) return 1;
+      own = true;
+    }
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      if (own) (void)hipStreamDestroy(cap);
+      return 1;
+    }
+    int rc = dm_batch_run(b, cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &g);
+    if (own) (void)hipStreamDestroy(cap);
+    if (rc != 0 || e != hipSuccess || g == nullptr) {
+      raise_error(nullptr, "dm_batch_graph_launch: stream capture failed");
+      return 1;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      drop_graph(b);
+      raise_error(nullptr, "dm_batch_graph_launch: graph instantiation failed");
+      return 1;
+    }
+    b->graph_stream = stream;
+  }
+  return hipGraphLaunch(b->exec, s) == hipSuccess ? 0 : 1;
+}
+
+extern "C" double dm_batch_autotune(DmBatch b, void *stream) {
+  if (b == nullptr || b->data.empty()) {
+    raise_error(nullptr, "dm_batch_autotune: batch not bound (dm_batch_bind)");
+    return -1.0;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  double best = 1e30;
+  int bt = b->tile_choice, br = b->radix_order;
+  for (int tile : {DM_BATCH_TILE_FULL, DM_BATCH_TILE_HALF})
+    for (int order : {DM_BATCH_RADIX_LARGE_FIRST, DM_BATCH_RADIX_SMALL_FIRST}) {
+      if (dm_batch_set_schedule(b, tile, order) != 0) continue;
+      dm_batch_run(b, s);  // warm-up
+      (void)hipEventRecord(e0, s);
+      const int reps = 3;
+      for (int r = 0; r < reps; r++) dm_batch_run(b, s);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms / reps < best) {
+        best = ms / reps;
+        bt = tile;
+        br = order;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (dm_batch_set_schedule(b, bt, br) != 0) return -1.0;
+  return best;
+}
+
+extern "C" int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice, int *radix_order) {
+  if (b == nullptr) return 1;
+  for (size_t i = 0; i < b->passes.size(); i++) {
+    if (tiles) tiles[i] = (long long)b->passes[i].tiles.size();
+    if (grid) grid[i] = b->passes[i].grid;
+  }
+  if (tile_choice) *tile_choice = b->tile_choice;
+  if (radix_order) *radix_order = b->radix_order;
+  return 0;
+}
+
+extern "C" double dm_batch_bytes(DmBatch b) {
+  if (b == nullptr) return 0;
+  double total = 0;
+  for (int i = 0; i < b->count; i++) {
+    double n = 1;
+    for (int d = 0; d < b->ndim; d++) n *= b->shapes[(size_t)i * b->ndim + d];
+    total += 2.0 * b->ndim * n * (b->dbl ? 16 : 8);
+  }
+  return total;
+}
+
+extern "C" void dm_batch_free(DmBatch b) {
+  if (b == nullptr) return;
+  release_bound(b);
+  delete b;
+}

@@ -1,0 +1,152 @@
+// K3: one axis pass of many independent local FFTs in one persistent launch.
+//
+// Each tile is nl lines of N points of one transform, read into an LDS tile
+// (coalesced: along the line for the contiguous axis, across the tile's
+// consecutive lines otherwise), transformed by the Stockham stages of its
+// schedule, and written back in place.  Tiles of different transforms (and
+// line lengths) share the launch; the workgroups loop over the tile list.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "fft_device.h"
+#include "kernels.h"
+
+namespace dm {
+namespace {
+
+using namespace fftdev;
+
+constexpr int BCAP_D = 4096;
+constexpr int BCAP_F = 4096;  // 8 points per thread in both precisions
+constexpr int BNTH = 512;
+
+// PRIMES: schedules with radix 5, 7, 11 or 13 (their butterflies need more
+// registers, so launches without them use a lighter kernel).
+template <typename R, bool INV, int CAP, int NTH, bool PRIMES>
+__global__ __launch_bounds__(NTH) void batch_pass_kernel(BatchLaunch p) {
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  constexpr int V = CAP / NTH;
+  const cx<R> *twall = reinterpret_cast<const cx<R> *>(p.twiddle);
+  for (long long t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
+    const BatchTile d = p.tiles[t];
+    const BatchSched &S = p.sched[d.sched];
+    const int N = S.N, T = S.T, nl = d.nl;
+    const int tot = T * N;
+    const int ls = lds_line(N);
+    const long long lstride = d.row ? (long long)N : 1LL;
+    cx<R> *data = reinterpret_cast<cx<R> *>(p.data[d.xform]) + d.base;
+    // load: addresses, then data, then LDS (branch-free; absent elements
+    // read element 0 of the tile and are dropped)
+    long long a[V];
+    int sl[V];
+    bool ok[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+      int e = threadIdx.x + k * NTH;
+      int ec = min(e, tot - 1);
+      int l, x;
+      if (d.row) {
+        l = fdiv(ec, S.m_n);
+        x = ec - l * N;
+      } else {
+        x = fdiv(ec, S.m_t);
+        l = ec - x * T;
+      }
+      ok[k] = e < tot && l < nl;
+      sl[k] = e < tot ? lds_index(l, x, ls) : -1;
+      a[k] = ok[k] ? (long long)l * lstride + (long long)x * d.ps : 0;
+    }
+    cx<R> v[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) v[k] = data[a[k]];
+#pragma unroll
+    for (int k = 0; k < V; k++)
+      if (sl[k] >= 0) buf[sl[k]] = ok[k] ? v[k] : cx<R>{0, 0};
+    __syncthreads();
+    const cx<R> *tw = twall + S.tw_off;
+    for (int s = 0; s < S.nstage; s++) {
+      switch (S.radix[s]) {
+        case 2: stage_lds<R, INV, 2, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 3: stage_lds<R, INV, 3, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 4: stage_lds<R, INV, 4, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 5: if constexpr (PRIMES) stage_lds<R, INV, 5, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 7: if constexpr (PRIMES) stage_lds<R, INV, 7, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 8: stage_lds<R, INV, 8, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 16: stage_lds<R, INV, 16, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        default: break;
+      }
+    }
+    // store: same element walk, straight from LDS
+#pragma unroll
+    for (int k = 0; k < V; k++)
+      if (ok[k]) data[a[k]] = buf[sl[k]];
+    __syncthreads();  // the next tile reuses the LDS tile
+  }
+}
+
+template <typename R, bool INV, int CAP, bool PRIMES>
+void launch_typed(const BatchLaunch &p, hipStream_t s) {
+  auto k = batch_pass_kernel<R, INV, CAP, BNTH, PRIMES>;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)p.grid), dim3(BNTH), p.lds, s, p);
+}
+
+}  // namespace
+
+int batch_tile_capacity(int dbl) { return dbl ? BCAP_D : BCAP_F; }
+
+size_t batch_lds_bytes(int dbl, int T, int N) {
+  return (size_t)T * (size_t)lds_line(N) * (dbl ? 16 : 8);
+}
+
+// Workgroups resident at once: by LDS (160 KiB per CU) and by registers
+// (512 VGPRs per SIMD lane, 4 SIMDs per CU, at most 8 waves per SIMD).
+long long batch_resident_groups(int dbl, int primes, size_t lds) {
+  const void *k;
+  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true>
+                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false>;
+  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true>
+                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false>;
+  hipFuncAttributes fa;
+  int vgprs = (hipFuncGetAttributes(&fa, k) == hipSuccess && fa.numRegs > 0) ? fa.numRegs : 128;
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (cus <= 0) cus = 256;
+  int waves_per_simd = std::min(8, 512 / (((vgprs + 7) / 8) * 8));
+  int by_regs = (waves_per_simd * 4) / (BNTH / 64);
+  int by_lds = (int)((160 * 1024) / std::max<size_t>(lds, 1));
+  return (long long)cus * std::max(1, std::min(by_regs, by_lds));
+}
+
+void launch_batch_pass(const BatchLaunch &p, void *stream) {
+  if (p.ntiles <= 0 || p.grid <= 0) return;
+  hipStream_t s = (hipStream_t)stream;
+  if (p.dbl) {
+    if (p.primes) {
+      if (p.inverse) launch_typed<double, true, BCAP_D, true>(p, s);
+      else launch_typed<double, false, BCAP_D, true>(p, s);
+    } else {
+      if (p.inverse) launch_typed<double, true, BCAP_D, false>(p, s);
+      else launch_typed<double, false, BCAP_D, false>(p, s);
+    }
+  } else {
+    if (p.primes) {
+      if (p.inverse) launch_typed<float, true, BCAP_F, true>(p, s);
+      else launch_typed<float, false, BCAP_F, true>(p, s);
+    } else {
+      if (p.inverse) launch_typed<float, true, BCAP_F, false>(p, s);
+      else launch_typed<float, false, BCAP_F, false>(p, s);
+    }
+  }
+}
+
+}  // namespace dm

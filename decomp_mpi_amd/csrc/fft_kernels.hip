@@ -14,199 +14,14 @@
 #include <cstdio>
 #include <cstdlib>
 
+#include "fft_device.h"
 #include "kernels.h"
 
 namespace dm {
 
 namespace {
 
-constexpr int NT = 256;  // threads per workgroup
-
-template <typename R>
-struct alignas(2 * sizeof(R)) cx {
-  R x, y;
-};
-template <typename R>
-__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
-template <typename R>
-__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
-template <typename R>
-__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
-  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-// multiply by -i (forward) or +i (inverse)
-template <typename R, bool INV>
-__device__ __forceinline__ cx<R> rot(cx<R> a) {
-  return INV ? cx<R>{-a.y, a.x} : cx<R>{a.y, -a.x};
-}
-
-// ---- butterflies: v[k] <- sum_n v[n] exp(-+2 pi i n k / RADIX) --------------
-
-template <typename R, bool INV>
-__device__ __forceinline__ void dft2(cx<R> *v) {
-  cx<R> a = v[0], b = v[1];
-  v[0] = a + b;
-  v[1] = a - b;
-}
-
-template <typename R, bool INV>
-__device__ __forceinline__ void dft4(cx<R> &a0, cx<R> &a1, cx<R> &a2, cx<R> &a3) {
-  cx<R> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = rot<R, INV>(a1 - a3);
-  a0 = t0 + t2;
-  a2 = t0 - t2;
-  a1 = t1 + t3;
-  a3 = t1 - t3;
-}
-
-template <typename R, bool INV>
-__device__ __forceinline__ void dft8(cx<R> *v) {
-  const R c = (R)0.7071067811865476;
-  cx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-  cx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-  dft4<R, INV>(e0, e1, e2, e3);
-  dft4<R, INV>(o0, o1, o2, o3);
-  // o_k *= w8^k
-  cx<R> w1, w3;
-  if (!INV) {
-    w1 = {(o1.x + o1.y) * c, (o1.y - o1.x) * c};
-    w3 = {(o3.y - o3.x) * c, -(o3.x + o3.y) * c};
-  } else {
-    w1 = {(o1.x - o1.y) * c, (o1.x + o1.y) * c};
-    w3 = {-(o3.x + o3.y) * c, (o3.x - o3.y) * c};
-  }
-  cx<R> w2 = rot<R, INV>(o2);
-  v[0] = e0 + o0;
-  v[4] = e0 - o0;
-  v[1] = e1 + w1;
-  v[5] = e1 - w1;
-  v[2] = e2 + w2;
-  v[6] = e2 - w2;
-  v[3] = e3 + w3;
-  v[7] = e3 - w3;
-}
-
-template <typename R, bool INV>
-__device__ __forceinline__ void dft16(cx<R> *v) {
-  // cos(pi k / 8), sin(pi k / 8), k = 0..7
-  const R c16[8] = {(R)1.0, (R)0.9238795325112867, (R)0.7071067811865476, (R)0.3826834323650898,
-                    (R)0.0, (R)-0.3826834323650898, (R)-0.7071067811865476, (R)-0.9238795325112867};
-  const R s16[8] = {(R)0.0, (R)0.3826834323650898, (R)0.7071067811865476, (R)0.9238795325112867,
-                    (R)1.0, (R)0.9238795325112867, (R)0.7071067811865476, (R)0.3826834323650898};
-  cx<R> e[8], o[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    e[k] = v[2 * k];
-    o[k] = v[2 * k + 1];
-  }
-  dft8<R, INV>(e);
-  dft8<R, INV>(o);
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    cx<R> w = {c16[k], INV ? s16[k] : -s16[k]};
-    cx<R> t = cmul(o[k], w);
-    v[k] = e[k] + t;
-    v[k + 8] = e[k] - t;
-  }
-}
-
-template <int P> struct Roots;
-template <> struct Roots<3> {
-  static constexpr double c[3] = {1.0, -0.5, -0.5};
-  static constexpr double s[3] = {0.0, 0.8660254037844386, -0.8660254037844386};
-};
-template <> struct Roots<5> {
-  static constexpr double c[5] = {1.0, 0.30901699437494745, -0.8090169943749475, -0.8090169943749475,
-                                  0.30901699437494745};
-  static constexpr double s[5] = {0.0, 0.9510565162951535, 0.5877852522924731, -0.5877852522924731,
-                                  -0.9510565162951535};
-};
-template <> struct Roots<7> {
-  static constexpr double c[7] = {1.0, 0.6234898018587335, -0.2225209339563144, -0.9009688679024191,
-                                  -0.9009688679024191, -0.2225209339563144, 0.6234898018587335};
-  static constexpr double s[7] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.4338837391175581,
-                                  -0.4338837391175581, -0.9749279121818236, -0.7818314824680298};
-};
-template <> struct Roots<11> {
-  static constexpr double c[11] = {1.0, 0.8412535328311812, 0.41541501300188644, -0.14231483827328514,
-                                   -0.6548607339452851, -0.9594929736144974, -0.9594929736144974,
-                                   -0.6548607339452851, -0.14231483827328514, 0.41541501300188644,
-                                   0.8412535328311812};
-  static constexpr double s[11] = {0.0, 0.5406408174555976, 0.9096319953545183, 0.9898214418809327,
-                                   0.7557495743542583, 0.28173255684142967, -0.28173255684142967,
-                                   -0.7557495743542583, -0.9898214418809327, -0.9096319953545183,
-                                   -0.5406408174555976};
-};
-template <> struct Roots<13> {
-  static constexpr double c[13] = {1.0, 0.8854560256532099, 0.5680647467311558, 0.12053668025532305,
-                                   -0.3546048870425356, -0.7485107481711011, -0.970941817426052,
-                                   -0.970941817426052, -0.7485107481711011, -0.3546048870425356,
-                                   0.12053668025532305, 0.5680647467311558, 0.8854560256532099};
-  static constexpr double s[13] = {0.0, 0.46472317204376856, 0.8229838658936564, 0.992708874098054,
-                                   0.9350162426854148, 0.6631226582407952, 0.23931566428755777,
-                                   -0.23931566428755777, -0.6631226582407952, -0.9350162426854148,
-                                   -0.992708874098054, -0.8229838658936564, -0.46472317204376856};
-};
-
-// Odd prime P: pair n with P-n so each output needs (P-1)/2 real
-// multiply-adds per part.
-template <typename R, bool INV, int P>
-__device__ __forceinline__ void dft_prime(cx<R> *v) {
-  constexpr int H = (P - 1) / 2;
-  cx<R> sum[H + 1], dif[H + 1];
-#pragma unroll
-  for (int n = 1; n <= H; n++) {
-    sum[n] = v[n] + v[P - n];
-    dif[n] = v[n] - v[P - n];
-  }
-  cx<R> a0 = v[0];
-  cx<R> x0 = a0;
-#pragma unroll
-  for (int n = 1; n <= H; n++) x0 = x0 + sum[n];
-#pragma unroll
-  for (int k = 1; k <= H; k++) {
-    cx<R> re = a0;  // a0 + sum_n cos * s_n
-    R imx = 0, imy = 0;  // sum_n sin * d_n
-#pragma unroll
-    for (int n = 1; n <= H; n++) {
-      int m = (n * k) % P;
-      R c = (R)Roots<P>::c[m], s = (R)Roots<P>::s[m];
-      re.x += c * sum[n].x;
-      re.y += c * sum[n].y;
-      imx += s * dif[n].x;
-      imy += s * dif[n].y;
-    }
-    // forward: X[k] = re - i * (imx + i imy) = (re.x + imy, re.y - imx)
-    if (!INV) {
-      v[k] = {re.x + imy, re.y - imx};
-      v[P - k] = {re.x - imy, re.y + imx};
-    } else {
-      v[k] = {re.x - imy, re.y + imx};
-      v[P - k] = {re.x + imy, re.y - imx};
-    }
-  }
-  v[0] = x0;
-}
-
-template <typename R, bool INV, int RADIX>
-__device__ __forceinline__ void dft(cx<R> *v) {
-  if constexpr (RADIX == 2) dft2<R, INV>(v);
-  else if constexpr (RADIX == 4) dft4<R, INV>(v[0], v[1], v[2], v[3]);
-  else if constexpr (RADIX == 8) dft8<R, INV>(v);
-  else if constexpr (RADIX == 16) dft16<R, INV>(v);
-  else dft_prime<R, INV, RADIX>(v);
-}
-
-// ---- fast division by runtime constants (numerators below 2^16) -------------
-
-__device__ __forceinline__ int fdiv(int n, unsigned long long magic) {
-  return (int)(((unsigned long long)(unsigned)n * magic) >> 32);
-}
-
-struct StageInfo {
-  unsigned long long m_nr;   // magic for N / radix
-  unsigned long long m_ns;   // magic for Ns
-  int nr, ns;
-};
+using namespace fftdev;
 
 struct Pass {
   FftLaunch p;
@@ -219,68 +34,9 @@ struct Pass {
   int tw_lds;                    // twiddle table copied to LDS
 };
 
-// LDS placement of point x of tile line `line`: one pad element every 16
-// points and an odd line stride in elements, so a column walk over
-// consecutive lines and strided Stockham writes spread over the banks.
-// (The tile plus a 512-point twiddle table stays under 80 KiB: two
-// workgroups per CU.)
-__host__ __device__ __forceinline__ int lds_pad(int x) { return x + (x >> 4); }
-__host__ __device__ __forceinline__ int lds_line(int n) { return n + (n >> 4) + 1; }
-__device__ __forceinline__ int lds_index(int line, int x, int ls) { return line * ls + lds_pad(x); }
-
 template <typename R, bool INV, int RADIX, int CAP, int NTH>
 __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx<R> *tw) {
-  constexpr int K = (CAP / RADIX + NTH - 1) / NTH;
-  const StageInfo &si = P.st[s];
-  const int nr = si.nr, ns = si.ns;
-  const int nb = P.p.T * nr;
-  cx<R> v[K][RADIX];
-  int dst[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    int b = threadIdx.x + k * NTH;
-    dst[k] = -1;
-    if (b < nb) {
-      int l = fdiv(b, si.m_nr);
-      int j = b - l * nr;
-      int jn = fdiv(j, si.m_ns);
-      int jj = j - jn * ns;
-      int base = l * P.ls;
-#pragma unroll
-      for (int r = 0; r < RADIX; r++) {
-        int x = j + r * nr;
-        v[k][r] = buf[base + lds_pad(x)];
-      }
-      if (ns > 1) {
-        // stage-ordered table: consecutive jj read consecutive entries
-        const cx<R> *tws = tw + (ns - 1) + jj;
-#pragma unroll
-        for (int r = 1; r < RADIX; r++) {
-          cx<R> w = tws[(r - 1) * ns];
-          if (INV) w.y = -w.y;
-          v[k][r] = cmul(v[k][r], w);
-        }
-      }
-      dft<R, INV, RADIX>(v[k]);
-      // line and Stockham-expanded index of the first output
-      int idxd = jn * ns * RADIX + jj;
-      dst[k] = (l << 16) | idxd;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    if (dst[k] >= 0) {
-      int l = dst[k] >> 16, idxd = dst[k] & 0xffff;
-      int base = l * P.ls;
-#pragma unroll
-      for (int r = 0; r < RADIX; r++) {
-        int x = idxd + r * ns;
-        buf[base + lds_pad(x)] = v[k][r];
-      }
-    }
-  }
-  __syncthreads();
+  stage_lds<R, INV, RADIX, CAP, NTH>(buf, P.st[s], P.p.T, P.ls, tw);
 }
 
 // Per-line address data of one side of a tile (in LDS).

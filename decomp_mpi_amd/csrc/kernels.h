@@ -31,6 +31,14 @@ struct ReshapeLaunch {
 };
 void launch_reshape(const ReshapeLaunch &p, void *stream);
 
+// Stage s of a line of N points: nr = N / radix butterflies per line, ns =
+// product of the earlier radices (Stockham span), magics for both.
+struct StageInfo {
+  unsigned long long m_nr;   // magic for N / radix
+  unsigned long long m_ns;   // magic for Ns
+  int nr, ns;
+};
+
 // ---- K2: batched 1D FFT pass -----------------------------------------------
 // Lines along axis t of an ndim array.  With equal strides on every rank
 // (uniform) point g lives at element offset sum_d abs[d][g_d] from the base
@@ -78,5 +86,39 @@ struct FftLaunch {
 void launch_fft_pass(const FftLaunch &p, void *stream);
 // Largest points per tile the FFT pass kernel holds for an element size.
 int fft_tile_capacity(int dbl, int column);
+
+// ---- K3: batched local FFT pass (many transforms, one axis of each) --------
+// A tile holds nl (<= T) lines of N points of one transform: point x of line
+// l at element base + l * (row ? N : 1) + x * ps of that transform's array.
+struct BatchTile {
+  long long base;
+  long long ps;
+  int xform;
+  int sched;
+  int nl;
+  int row;
+};
+// Per (line length, tile height, radix order): stages and twiddles.
+struct BatchSched {
+  int N, T, nstage, tw_off;    // tw_off: start of the stage-ordered roots
+  int radix[16];
+  unsigned long long m_n, m_t;
+  StageInfo st[16];
+};
+struct BatchLaunch {
+  const BatchTile *tiles;
+  long long ntiles;
+  const BatchSched *sched;
+  void *const *data;           // device array: transform i's data pointer
+  const void *twiddle;         // all schedules' roots in the pass precision
+  int dbl, inverse;
+  int primes;                  // some schedule has a radix 5, 7, 11 or 13
+  long long grid;              // persistent workgroups (tile loop inside)
+  size_t lds;
+};
+void launch_batch_pass(const BatchLaunch &p, void *stream);
+int batch_tile_capacity(int dbl);
+size_t batch_lds_bytes(int dbl, int T, int N);
+long long batch_resident_groups(int dbl, int primes, size_t lds);
 
 }  // namespace dm

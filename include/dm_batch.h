@@ -1,0 +1,56 @@
+/*
+ * dm_batch.h -- many independent local multi-dimensional FFTs at once.
+ *
+ * No counterpart in the reference, whose transforms are one distributed
+ * array at a time (split_fft.h).  This is the batched form: transforms of
+ * uneven shapes (each a contiguous C-order array in device memory, last
+ * axis fastest) are transformed in place by one persistent launch per axis,
+ * every launch covering the tiles of all transforms.  The launches can be
+ * captured once into a HIP graph and replayed.
+ *
+ * Precision: 1 complex double, 0 complex float (twiddles generated in
+ * double precision either way).  Direction FFTW_FORWARD (-1) or
+ * FFTW_BACKWARD (+1), unnormalised, as everywhere in this library.
+ */
+#ifndef DM_BATCH_H
+#define DM_BATCH_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dm_batch *DmBatch;
+
+/* Tile and radix schedule choices (dm_batch_set_schedule, dm_batch_autotune). */
+#define DM_BATCH_TILE_FULL 0    /* tiles of up to the kernel's capacity */
+#define DM_BATCH_TILE_HALF 1    /* tiles of up to half of it */
+#define DM_BATCH_RADIX_LARGE_FIRST 0
+#define DM_BATCH_RADIX_SMALL_FIRST 1
+
+/** count transforms of ndim (1..3) axes; shapes[i * ndim + d] is axis d of
+ *  transform i.  Axis lengths must factor into 2, 3, 5, 7, 11, 13 with at
+ *  most 16 stages.  Returns NULL (after an error message) otherwise. */
+DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int precision, int direction);
+/** Device pointers of the transforms' data, data[i] for transform i.  Builds
+ *  and uploads the tile tables; must precede run, graph and autotune. */
+int dm_batch_bind(DmBatch b, void *const *data);
+/** Tile and radix schedule (the defaults: FULL, LARGE_FIRST); rebinds. */
+int dm_batch_set_schedule(DmBatch b, int tile, int radix_order);
+/** Enqueue the transform on a HIP stream (NULL: the legacy default stream). */
+int dm_batch_run(DmBatch b, void *stream);
+/** First call: capture dm_batch_run into a HIP graph; every call launches
+ *  the graph on the stream.  Rebinding or rescheduling drops the graph. */
+int dm_batch_graph_launch(DmBatch b, void *stream);
+/** Times every tile/radix schedule on the bound data (which it overwrites)
+ *  and keeps the fastest; returns its milliseconds per run (< 0: error). */
+double dm_batch_autotune(DmBatch b, void *stream);
+/** Tiles per axis pass and launch grid of the current binding (inspection). */
+int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice, int *radix_order);
+/** Algorithmic HBM bytes of one run: sum over transforms of 2 * ndim * N * elem. */
+double dm_batch_bytes(DmBatch b);
+void dm_batch_free(DmBatch b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
