@@ -99,34 +99,38 @@ extern "C" DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int pre
   return b;
 }
 
-static bool bind_tables(DmBatch b) {
-  release_bound(b);
-  const int cap = dm::batch_tile_capacity(b->dbl);
-  // schedules per (N, T): radix list, magics, twiddle block
-  std::map<std::pair<int, int>, int> sched_of;
-  std::vector<dm::BatchSched> sched;
-  std::vector<double> tw;  // complex doubles, converted for FP32 below
+namespace dm {
+
+// Host image of a batch: tiles per axis pass (last axis first), schedules
+// and their stage-ordered twiddles (complex double), LDS need, and whether a
+// radix 5, 7, 11 or 13 occurs.
+bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_choice, int radix_order,
+                      BatchHost *out) {
+  BatchHost &h = *out;
+  h = BatchHost();
+  const int cap = batch_tile_capacity(dbl);
+  std::map<std::pair<int, int>, int> sched_of;  // (N, T) -> schedule
   auto schedule = [&](int N, int T) -> int {
     auto key = std::make_pair(N, T);
     auto it = sched_of.find(key);
     if (it != sched_of.end()) return it->second;
-    dm::BatchSched S{};
+    BatchSched S{};
     std::vector<int> rad;
-    dm::choose_radices(N, &rad);
-    if (b->radix_order == DM_BATCH_RADIX_SMALL_FIRST) std::reverse(rad.begin(), rad.end());
+    choose_radices(N, &rad);
+    if (radix_order == DM_BATCH_RADIX_SMALL_FIRST) std::reverse(rad.begin(), rad.end());
     S.N = N;
     S.nstage = (int)rad.size();
     S.T = T;
     S.m_n = magic(N);
     S.m_t = magic(T);
     int ns = 1;
-    S.tw_off = (int)(tw.size() / 2);
-    std::vector<double> root = dm::roots(N);
+    S.tw_off = (int)(h.tw.size() / 2);
+    std::vector<double> root = roots(N);
     std::vector<double> block(2 * (size_t)N, 0.0);
     for (int s = 0; s < S.nstage; s++) {
       int R = rad[s];
       S.radix[s] = R;
-      if (R == 5 || R == 7 || R == 11 || R == 13) b->primes = 1;
+      if (R == 5 || R == 7 || R == 11 || R == 13) h.primes = 1;
       S.st[s].nr = N / R;
       S.st[s].ns = ns;
       S.st[s].m_nr = magic(N / R);
@@ -141,34 +145,33 @@ static bool bind_tables(DmBatch b) {
         }
       ns *= R;
     }
-    tw.insert(tw.end(), block.begin(), block.end());
-    int id = (int)sched.size();
-    sched.push_back(S);
+    h.tw.insert(h.tw.end(), block.begin(), block.end());
+    int id = (int)h.sched.size();
+    h.sched.push_back(S);
     sched_of[key] = id;
-    b->lds = std::max(b->lds, dm::batch_lds_bytes(b->dbl, T, N));
+    h.lds = std::max(h.lds, batch_lds_bytes(dbl, T, N));
     return id;
   };
-  b->lds = 0;
-  b->primes = 0;
-  b->passes.assign(b->ndim, PassTables());
-  for (int i = 0; i < b->count; i++) {
-    const int *n = &b->shapes[(size_t)i * b->ndim];
-    for (int d = 0; d < b->ndim; d++) {
+  h.tiles.assign(ndim, std::vector<BatchTile>());
+  for (int i = 0; i < count; i++) {
+    const int *n = &shapes[(size_t)i * ndim];
+    for (int d = 0; d < ndim; d++) {
       long long inner = 1, outer = 1;
-      for (int e = d + 1; e < b->ndim; e++) inner *= n[e];
+      for (int e = d + 1; e < ndim; e++) inner *= n[e];
       for (int e = 0; e < d; e++) outer *= n[e];
       const int N = n[d];
+      if (N < 1 || N > cap) return false;
       const bool row = inner == 1;
       // lines per tile: as many as the tile holds; a column walk never more
       // than the contiguous run (so no tile is mostly padding)
-      const int limit = b->tile_choice == DM_BATCH_TILE_HALF ? cap / 2 : cap;
+      const int limit = tile_choice == DM_BATCH_TILE_HALF ? cap / 2 : cap;
       int T = std::max(1, limit / N);
       if (!row) T = (int)std::min<long long>(T, inner);
       int sid = schedule(N, T);
-      auto &tl = b->passes[b->ndim - 1 - d].tiles;  // passes run last axis first
+      auto &tl = h.tiles[ndim - 1 - d];  // passes run last axis first
       if (row) {
         for (long long o = 0; o < outer; o += T) {
-          dm::BatchTile t{};
+          BatchTile t{};
           t.base = o * N;
           t.ps = 1;
           t.xform = i;
@@ -180,7 +183,7 @@ static bool bind_tables(DmBatch b) {
       } else {
         for (long long o = 0; o < outer; o++)
           for (long long c = 0; c < inner; c += T) {
-            dm::BatchTile t{};
+            BatchTile t{};
             t.base = o * N * inner + c;
             t.ps = inner;
             t.xform = i;
@@ -192,6 +195,24 @@ static bool bind_tables(DmBatch b) {
       }
     }
   }
+  return true;
+}
+
+}  // namespace dm
+
+static bool bind_tables(DmBatch b) {
+  release_bound(b);
+  dm::BatchHost h;
+  if (!dm::build_batch_host(b->count, b->ndim, b->shapes.data(), b->dbl, b->tile_choice, b->radix_order, &h)) {
+    raise_error(nullptr, "dm_batch_bind: unsupported axis length");
+    return false;
+  }
+  b->lds = h.lds;
+  b->primes = h.primes;
+  b->passes.assign(b->ndim, PassTables());
+  for (int d = 0; d < b->ndim; d++) b->passes[d].tiles = std::move(h.tiles[d]);
+  std::vector<dm::BatchSched> &sched = h.sched;
+  std::vector<double> &tw = h.tw;
   // upload
   auto up = [&](const void *src, size_t bytes) -> void * {
     void *d = dm::device_alloc(std::max<size_t>(bytes, 16));

@@ -2,6 +2,7 @@
 // Plain structs of device pointers and sizes; no runtime types.
 #pragma once
 #include <cstdint>
+#include <vector>
 
 #include "internal.h"
 
@@ -117,6 +118,17 @@ struct BatchLaunch {
   size_t lds;
 };
 void launch_batch_pass(const BatchLaunch &p, void *stream);
+// Host tables of a batch (batch.cpp): tiles[pass] (last axis first),
+// schedules, their stage-ordered roots as complex doubles, LDS bytes.
+struct BatchHost {
+  std::vector<std::vector<BatchTile>> tiles;
+  std::vector<BatchSched> sched;
+  std::vector<double> tw;
+  size_t lds = 0;
+  int primes = 0;
+};
+bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_choice, int radix_order,
+                      BatchHost *out);
 int batch_tile_capacity(int dbl);
 size_t batch_lds_bytes(int dbl, int T, int N);
 long long batch_resident_groups(int dbl, int primes, size_t lds);
