@@ -121,6 +121,157 @@ def reshape_chain(C, dm, comm, reps=10):
             "frac_of_peak": [round(v / PEAK_HBM_GBS, 4) for v in gbs]}
 
 
+def batched_bench(C, dm, steps, warmup):
+    """256 independent complex-float 3D FFTs of seeded uneven shapes (each
+    axis from {32, 48, 64, 96, 128}) in place, one batched launch per axis
+    replayed from a HIP graph; schedule from dm_batch_autotune."""
+    import ctypes
+    from decomp_mpi_amd import workloads
+    shapes = workloads.batched_shapes(256, seed=4)
+    comm = C.dm_world_create(1)
+    stream = C.dm_get_stream(comm)
+    offs, at = [], 0
+    for sh in shapes:
+        offs.append(at)
+        at += (int(np.prod(sh)) * 8 + 255) // 256 * 256
+    buf = dm.DistBuffer(comm, at)
+    rng = np.random.default_rng(6)
+    xs = [workloads.uniform_complex(rng, sh, 0) for sh in shapes]
+    plan = C.dm_batch_plan(len(shapes), 3, dm.ints([v for sh in shapes for v in sh]), 0, dm.FFTW_FORWARD)
+    ptrs = (ctypes.c_void_p * len(shapes))(*[buf.ptr + o for o in offs])
+    if not plan or C.dm_batch_bind(plan, ptrs) != 0:
+        raise RuntimeError("batched plan failed")
+    tune_ms = C.dm_batch_autotune(plan, stream)
+    tile, order = ctypes.c_int(), ctypes.c_int()
+    C.dm_batch_info(plan, None, None, ctypes.byref(tile), ctypes.byref(order))
+
+    def upload():
+        for x, o in zip(xs, offs):
+            buf.put(0, x, o)
+    upload()
+    C.dm_batch_graph_launch(plan, stream)
+    C.dm_synchronize(comm)
+    max_rel, max_abs = 0.0, 0.0
+    for x, o, sh in zip(xs, offs, shapes):
+        got = buf.get(0, x.nbytes, np.complex64, o).reshape(sh).astype(np.complex128)
+        want = np.fft.fftn(x.astype(np.complex128))
+        max_rel = max(max_rel, float(np.linalg.norm(got - want) / np.linalg.norm(want)))
+        max_abs = max(max_abs, float(np.max(np.abs(got - want)) / np.max(np.abs(want))))
+    for _ in range(warmup):
+        C.dm_batch_graph_launch(plan, stream)
+    C.dm_synchronize(comm)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        C.dm_batch_graph_launch(plan, stream)
+    C.dm_synchronize(comm)
+    dt = (time.perf_counter() - t0) / steps
+    flops = sum(5 * np.prod(sh) * np.log2(np.prod(sh)) for sh in shapes)
+    alg = C.dm_batch_bytes(plan)
+    C.dm_batch_free(plan)
+    buf.free()
+    C.dm_world_free(comm)
+    return {"workload": "256 complex-float 3D FFTs, axes from {32,48,64,96,128} (seed 4), in place, "
+                        "one HIP graph of 3 batched launches",
+            "ms_per_batch": round(dt * 1e3, 4), "transforms_per_s": round(len(shapes) / dt, 1),
+            "GFLOP_per_s": round(flops / dt / 1e9, 1), "GB_per_s": round(alg / dt / 1e9, 1),
+            "frac_of_peak": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
+            "max_rel_l2_vs_fp64": max_rel, "max_abs_err_over_max_vs_fp64": max_abs,
+            "schedule": {"tile": ["full", "half"][tile.value], "radix_order": ["large_first", "small_first"][order.value],
+                         "autotune_best_ms": round(tune_ms, 4)}}
+
+
+def serving_bench(C, dm, count=1000, streams=4, max_batch=8, budget=64 << 30):
+    """Poisson arrivals of mixed 3D FFT requests (axes log-uniform in
+    [32, 256] rounded to 16; 70% complex float, 30% complex double; random
+    direction) through dm_serve with host (pinned) input and output, at 60%
+    of the closed-loop capacity measured first.  Latency = finish - arrival."""
+    import ctypes
+    from decomp_mpi_amd import workloads
+    probe = workloads.serving_requests(count, 1.0, seed=5)   # shapes; arrivals rescaled below
+    ring = 32
+    sizes = [0] * ring
+    for i, (_, sh, prec, _) in enumerate(probe):
+        sizes[i % ring] = max(sizes[i % ring], int(np.prod(sh)) * (16 if prec else 8))
+    bufs = [C.dm_host_alloc(sz) for sz in sizes]
+    if not all(bufs):
+        raise MemoryError("pinned host buffers")
+    rng = np.random.default_rng(8)
+    for b, sz in zip(bufs, sizes):   # inputs: uniform[-1,1) values
+        v = rng.uniform(-1, 1, sz // 4).astype(np.float32)
+        ctypes.memmove(b, v.ctypes.data, sz)
+
+    def run(reqs, arrivals):
+        s = C.dm_server_create(budget, streams, max_batch)
+        owner = [-1] * ring
+        ids = []
+        t_start = time.perf_counter()
+        for i, ((_, sh, prec, d), t_arr) in enumerate(zip(reqs, arrivals)):
+            while time.perf_counter() - t_start < t_arr:
+                C.dm_server_poll(s)
+            j = i % ring
+            if owner[j] >= 0:
+                C.dm_server_wait(s, owner[j])
+            rid = C.dm_server_submit(s, 3, dm.ints(list(sh)), prec, d, bufs[j], bufs[j])
+            owner[j] = rid
+            ids.append(rid)
+            C.dm_server_poll(s)
+        C.dm_server_drain(s)
+        sub, fin = ctypes.c_double(), ctypes.c_double()
+        times = []
+        for rid in ids:
+            C.dm_server_times(s, rid, ctypes.byref(sub), None, ctypes.byref(fin))
+            times.append((sub.value, fin.value))
+        nb, mx = ctypes.c_longlong(), ctypes.c_longlong()
+        peak = ctypes.c_size_t()
+        C.dm_server_stats(s, ctypes.byref(nb), ctypes.byref(mx), ctypes.byref(peak), None)
+        C.dm_server_destroy(s)
+        return times, nb.value, mx.value, peak.value
+
+    # closed loop: everything at time 0
+    n0 = min(256, count)
+    times, _, _, _ = run(probe[:n0], [0.0] * n0)
+    capacity = n0 / (max(f for _, f in times) - min(sub for sub, _ in times))
+    lam = 0.6 * capacity
+    reqs = workloads.serving_requests(count, lam, seed=5)
+    arrivals = [a for a, _, _, _ in reqs]
+    times, batches, max_seen, peak = run(reqs, arrivals)
+    first = min(sub for sub, _ in times)
+    lat = np.array([fin - (first + arr) for (sub, fin), arr in zip(times, arrivals)]) * 1e3
+    span = max(f for _, f in times) - first
+    # accuracy per precision class on a separate mixed set
+    acc = {0: 0.0, 1: 0.0}
+    s = C.dm_server_create(budget, streams, max_batch)
+    checks = []
+    for (_, sh, prec, d) in workloads.serving_requests(24, 1.0, seed=9):
+        sh = tuple(min(a, 128) for a in sh)
+        x = workloads.uniform_complex(rng, sh, prec)
+        pin = C.dm_host_alloc(x.nbytes)
+        ctypes.memmove(pin, x.ctypes.data, x.nbytes)
+        checks.append((x, pin, d, C.dm_server_submit(s, 3, dm.ints(list(sh)), prec, d, pin, pin)))
+        C.dm_server_poll(s)
+    C.dm_server_drain(s)
+    for x, pin, d, _ in checks:
+        got = np.frombuffer((ctypes.c_char * x.nbytes).from_address(pin), dtype=x.dtype).reshape(x.shape)
+        x64 = x.astype(np.complex128)
+        want = np.fft.fftn(x64) if d == -1 else np.fft.ifftn(x64) * x64.size
+        prec = 1 if x.dtype == np.complex128 else 0
+        acc[prec] = max(acc[prec], float(np.linalg.norm(got.astype(np.complex128) - want) / np.linalg.norm(want)))
+        C.dm_host_free(pin)
+    C.dm_server_destroy(s)
+    for b in bufs:
+        C.dm_host_free(b)
+    return {"workload": f"{count} Poisson requests, axes log-uniform [32,256] step 16, 70% c64 / 30% c128, "
+                        f"random direction, pinned host in/out, budget {budget >> 30} GiB, {streams} streams, "
+                        f"batches of up to {max_batch}",
+            "closed_loop_capacity_req_per_s": round(capacity, 1), "lambda_req_per_s": round(lam, 1),
+            "throughput_req_per_s": round(count / span, 1),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "mean": round(float(lat.mean()), 3)},
+            "batches": batches, "largest_batch": max_seen, "peak_device_bytes": peak,
+            "max_rel_l2_vs_fp64": {"c64": acc[0], "c128": acc[1]}}
+
+
 def measured_traffic(bytes_per_launch):
     """HBM bytes per FFT pass launch from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what it
@@ -189,6 +340,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-reshape", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--factors", type=str, default=None,
                     help="B factors b0,b1,b2 for makeParallelPlan instead of the autotuner (profiling runs)")
     args = ap.parse_args()
@@ -224,11 +377,13 @@ def main():
     pass_ms = kern / passes * 1e3
     pass_bytes = 2 * N * 16
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
-    # forward result for the FP32 error figure
+    # forward result for the FP32 error figure, in frequency order (the two
+    # precisions' autotuned plans may place the output differently)
+    from decomp_mpi_amd.layout import flat_index
     dat.put_all(host_x)
     C.runSplitPlan(plan, dm.FFTW_FORWARD)
     ext = sp.extent
-    ref_fwd = work.get_all(ext * 16, np.complex128)
+    ref_fwd = work.get_all(ext * 16, np.complex128).ravel()[flat_index(C.getTransformed(plan), ext)]
     C.freeSplitPlan(plan)
     dat.free()
     work.free()
@@ -243,10 +398,9 @@ def main():
         dat32.put_all(host_x.astype(np.complex64))
         C.runSplitPlan(plan32, dm.FFTW_FORWARD)
         ext32 = sp32.extent
-        got32 = work32.get_all(ext32 * 8, np.complex64)
-        err32 = None
-        if A32 == A64 and ext32 == ext:
-            err32 = float(np.linalg.norm(got32.astype(np.complex128) - ref_fwd) / np.linalg.norm(ref_fwd))
+        got32 = work32.get_all(ext32 * 8, np.complex64).ravel()[flat_index(C.getTransformed(plan32), ext32)]
+        err32 = float(np.linalg.norm(got32.astype(np.complex128) - ref_fwd) / np.linalg.norm(ref_fwd))
+        del got32
         C.runSplitPlan(plan32, dm.FFTW_BACKWARD)
         back32 = dat32.get(0, per * 8, np.complex64).astype(np.complex128) / N
         rt32 = float(np.linalg.norm(back32 - host_x[0]) / np.linalg.norm(host_x[0]))
@@ -271,6 +425,15 @@ def main():
         print("bench: timing the 512^3 int64 reshape chain", file=sys.stderr, flush=True)
         chain = reshape_chain(C, dm, comm)
 
+    batched = None
+    if not args.no_batched:
+        print("bench: batched 256 uneven FFTs", file=sys.stderr, flush=True)
+        batched = batched_bench(C, dm, max(args.steps, 5), max(args.warmup, 1))
+    serving = None
+    if not args.no_serving:
+        print("bench: serving loop", file=sys.stderr, flush=True)
+        serving = serving_bench(C, dm)
+
     cpu = None if args.no_cpu else cpu_baseline()
     value = alg_step / (wall / args.steps) / 1e9
     out = {
@@ -289,7 +452,8 @@ def main():
         "config": {"workload": "split FFT forward+inverse, 512^3 complex double, pencil start "
                                "{Local(512), Block(512,4) rs8, Block(512,8) rs1} on 32 virtual ranks",
                    "A_factors": A64, "gflops": round(2 * 5 * N * np.log2(N) / (wall / args.steps) / 1e9, 1),
-                   "round_trip_rel_l2": rt64, "fp32_form": fp32, "reshape_chain": chain},
+                   "round_trip_rel_l2": rt64, "fp32_form": fp32, "reshape_chain": chain,
+                   "batched": batched, "serving": serving},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": (measured_traffic(pass_bytes) or {}).get("bytes"),

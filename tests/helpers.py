@@ -95,37 +95,6 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-MAPPER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int)
 
-
-def layout_tables(dptr):
-    """Per-axis owner-position and local-index tables of a product
-    decomposition, with its rank and memory strides (the view of the rank
-    that built it; layouts used here have equal strides on every rank)."""
-    d = dm.decomp(dptr)
-    out = []
-    for i in range(d.ndim):
-        ax = d.dims[i].contents
-        dd = ax.decomp.contents
-        p = ctypes.cast(ax.decomp, ctypes.c_void_p).value
-        ro, to = MAPPER(dd.rank_offset), MAPPER(dd.type_offset)
-        rk = np.array([ro(p, g) for g in range(dd.coord_size)], np.int64)
-        of = np.array([to(p, g) for g in range(dd.coord_size)], np.int64)
-        out.append((rk, of, ax.rank_stride, ax.memory_stride))
-    return out, d.proc_used
-
-
-def flat_index(dptr, slot):
-    """Element index (rank * slot + local offset) of every global coordinate,
-    as an int64 array of the global shape; -1 where no rank owns it."""
-    tabs, proc_used = layout_tables(dptr)
-    n = len(tabs)
-    rank = np.zeros([1] * n, np.int64)
-    off = np.zeros([1] * n, np.int64)
-    for i, (rk, of, rs, ms) in enumerate(tabs):
-        shape = [1] * n
-        shape[i] = len(rk)
-        rank = rank + (rk * rs).reshape(shape)
-        off = off + (of * ms).reshape(shape)
-    idx = rank * slot + off
-    return np.where(rank < proc_used, idx, -1)
+# result placement helpers live in the package (bench.py uses them too)
+from decomp_mpi_amd.layout import MAPPER, flat_index, layout_tables  # noqa: E402,F401
