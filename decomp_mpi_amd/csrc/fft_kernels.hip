@@ -272,6 +272,7 @@ __global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
       tile = tile - r + (r % 8) * P.xcd_pair + r / 8;
     }
   }
+  tile += p.tile_base;
   const int nl = setup_lines<UIN, UOUT>(p, tile, lin, lout, T, NTH);
   // twiddle table in LDS after the line data
   size_t tw_at = (size_t)T * P.ls * sizeof(cx<R>) + LineAddr<UIN>::bytes(T, p.ndim) +
@@ -507,8 +508,11 @@ void launch_fft_pass(const FftLaunch &p, void *stream) {
   }
   long long batch = 1;
   for (int k = 0; k < p.nbatch; k++) batch *= p.L[p.batch_axis[k]];
-  P.p.nblocks = (long long)p.tiles_u * batch;
-  if (P.p.nblocks <= 0) return;
+  if (p.nblocks <= 0) {  // the whole pass
+    P.p.nblocks = (long long)p.tiles_u * batch;
+    P.p.tile_base = 0;
+  }
+  if (P.p.nblocks <= 0 || P.p.tile_base < 0 || P.p.tile_base + P.p.nblocks > (long long)p.tiles_u * batch) return;
   size_t lds = fft_lds_bytes(p);
   hipStream_t s = (hipStream_t)stream;
   if (p.dbl) {

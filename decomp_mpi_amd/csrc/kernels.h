@@ -83,7 +83,10 @@ struct FftLaunch {
   int dbl;                    // 1: complex double, 0: complex float
   const void *in;
   void *out;
+  long long tile_base;        // this launch runs tiles [tile_base, tile_base + nblocks)
+  int pass_index;             // host bookkeeping: the pass this launch belongs to
 };
+// nblocks == 0: the launch covers every tile of the pass.
 void launch_fft_pass(const FftLaunch &p, void *stream);
 // Largest points per tile the FFT pass kernel holds for an element size.
 int fft_tile_capacity(int dbl, int column);

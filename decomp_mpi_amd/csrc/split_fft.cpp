@@ -204,8 +204,10 @@ static void program_release(dm_fft_program &p) {
   p.passes.clear();
 }
 
-// Build one direction's passes.  Forward: dat (start) -> ... -> W0 (output);
-// backward: W0 -> ... -> dat.  Intermediates alternate between W0 and W1.
+// Build one direction's passes.  Forward: dat (start) -> W0 -> ... -> W0
+// (output); backward: W0 -> ... -> W0 -> dat.  The intermediate passes work
+// in place: every tile reads all of its points before writing them, tiles
+// are disjoint, and an intermediate pass reads and writes the same layout.
 static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, const dm::Side &O, void *dat,
                               void *w0, void *w1, dm_fft_program *prog) {
   dm_split_ext *x = plan->ext;
@@ -231,14 +233,15 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
   for (int i = 0; i < n; i++) {
     void *bin, *bout;
     const dm::Side *si, *so;
+    (void)w1;  // the workspace's second half stays free (the reference's 2 x extent contract)
     if (!inverse) {
-      bout = ((n - 1 - i) % 2 == 0) ? w0 : w1;
+      bout = w0;
       bin = (i == 0) ? dat : prev;
       si = (i == 0) ? &S : &O;
       so = &O;
     } else {
       bin = (i == 0) ? w0 : prev;
-      bout = (i == n - 1) ? dat : ((i % 2 == 0) ? w1 : w0);
+      bout = (i == n - 1) ? dat : w0;
       si = &O;
       so = (i == n - 1) ? &S : &O;
     }
@@ -247,7 +250,9 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
   }
   prog->mem = dm::finalize_passes(plan->comm, tb, pp, &prog->passes);
   prog->inverse = inverse ? 1 : 0;
-  return prog->mem != nullptr;
+  if (prog->mem == nullptr) return false;
+  for (size_t i = 0; i < prog->passes.size(); i++) prog->passes[i].pass_index = (int)i;
+  return true;
 }
 
 static bool compile_plan(SplitPlan plan) {
@@ -611,7 +616,7 @@ static void run_program(SplitPlan plan, dm_fft_program &prog, int t_first, int t
   for (size_t i = 0; i < n; i++) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
-    plan->times[i == 0 ? t_first : t_rest] += ms * 1e-3;
+    plan->times[prog.passes[i].pass_index == 0 ? t_first : t_rest] += ms * 1e-3;
     total += ms * 1e-3;
   }
   plan->times[ALL] += total;
