@@ -200,8 +200,10 @@ __device__ __forceinline__ int lds_index(int line, int x, int ls) { return line 
 // One Stockham stage over the T lines of an LDS tile (line stride ls):
 // butterflies read points j + r * nr of their line, get the stage twiddles
 // (stage-ordered table), transform, and write back to their Stockham places.
-template <typename R, bool INV, int RADIX, int CAP, int NTH>
-__device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T, int ls, const cx<R> *tw) {
+// TW: pointer to the twiddles, typed by where they live (an LDS-qualified
+// pointer reads with LDS instructions; a generic one would use flat loads).
+template <typename R, bool INV, int RADIX, int CAP, int NTH, typename TW>
+__device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T, int ls, TW tw) {
   constexpr int K = (CAP / RADIX + NTH - 1) / NTH;
   const int nr = si.nr, ns = si.ns;
   const int nb = T * nr;
@@ -224,10 +226,10 @@ __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T
       }
       if (ns > 1) {
         // stage-ordered table: consecutive jj read consecutive entries
-        const cx<R> *tws = tw + (ns - 1) + jj;
+        TW tws = tw + (ns - 1) + jj;
 #pragma unroll
         for (int r = 1; r < RADIX; r++) {
-          cx<R> w = tws[(r - 1) * ns];
+          cx<R> w = {tws[(r - 1) * ns].x, tws[(r - 1) * ns].y};
           if (INV) w.y = -w.y;
           v[k][r] = cmul(v[k][r], w);
         }

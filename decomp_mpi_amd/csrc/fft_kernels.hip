@@ -34,8 +34,8 @@ struct Pass {
   int tw_lds;                    // twiddle table copied to LDS
 };
 
-template <typename R, bool INV, int RADIX, int CAP, int NTH>
-__device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx<R> *tw) {
+template <typename R, bool INV, int RADIX, int CAP, int NTH, typename TW>
+__device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, TW tw) {
   stage_lds<R, INV, RADIX, CAP, NTH>(buf, P.st[s], P.p.T, P.ls, tw);
 }
 
@@ -43,14 +43,22 @@ __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, const cx
 //   uniform:     base[l] = sum over axes other than t of abs[d][g_d]
 //   non-uniform: rbase[l] = owner contribution of those axes, offs[l][k] =
 //                their local offsets (k over the axes other than t)
+// Pointers into LDS are typed as such (address space 3) so that reads of
+// the line data are LDS instructions; through a generic pointer they would
+// be flat loads, queued in order with the global data loads.
+typedef __attribute__((address_space(3))) i64 lds_i64;
+typedef __attribute__((address_space(3))) int lds_int;
+template <typename R>
+using lds_cx = __attribute__((address_space(3))) cx<R>;
+
 template <bool UNIFORM>
 struct LineAddr;
 
 template <>
 struct LineAddr<true> {
-  i64 *base;
+  lds_i64 *base;
   __host__ __device__ static size_t bytes(int T, int) { return (size_t)T * sizeof(i64); }
-  __device__ void init(char *mem, int, int) { base = reinterpret_cast<i64 *>(mem); }
+  __device__ void init(char *mem, int, int) { base = (lds_i64 *)(mem); }
   __device__ void set_line(const FftLaunch &p, const i64 *const *abs, const int *g, int l) {
     i64 a = 0;
     for (int d = 0; d < p.ndim; d++)
@@ -64,12 +72,12 @@ struct LineAddr<true> {
 
 template <>
 struct LineAddr<false> {
-  int *rbase;
-  int *offs;
+  lds_int *rbase;
+  lds_int *offs;
   int nd;
   __host__ __device__ static size_t bytes(int T, int ndim) { return (size_t)T * sizeof(int) * (1 + ndim); }
   __device__ void init(char *mem, int T, int ndim) {
-    rbase = reinterpret_cast<int *>(mem);
+    rbase = (lds_int *)(mem);
     offs = rbase + T;
     nd = ndim;
   }
@@ -202,9 +210,9 @@ __device__ __forceinline__ void load_first_stage(cx<R> *buf, const Pass &P, cons
 
 // Last stage straight to memory: read butterfly inputs from LDS, twiddle,
 // transform, and store outputs j + nr*r of line l.
-template <typename R, bool INV, int RADIX, int CAP, int NTH, bool UOUT>
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool UOUT, typename TW>
 __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P, const LineAddr<UOUT> &lout,
-                                                 cx<R> *out, const i64 *outT, const cx<R> *tw, int nl) {
+                                                 cx<R> *out, const i64 *outT, TW tw, int nl) {
   constexpr int K = (CAP / NTH) / RADIX;
   const FftLaunch &p = P.p;
   const StageInfo &si = P.st[p.nstage - 1];
@@ -225,7 +233,8 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
     // in the last stage ns = nr, so j % ns = j (stage-ordered table)
 #pragma unroll
     for (int r = 1; r < RADIX; r++) {
-      cx<R> w = tw[(nr - 1) + (r - 1) * nr + j];
+      const int ti = (nr - 1) + (r - 1) * nr + j;
+      cx<R> w = {tw[ti].x, tw[ti].y};
       if (INV) w.y = -w.y;
       v[q][r] = cmul(v[q][r], w);
     }
@@ -288,7 +297,6 @@ __global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
   const i64 *outT = p.off_out[p.t];
   const cx<R> *in = reinterpret_cast<const cx<R> *>(p.in);
   cx<R> *out = reinterpret_cast<cx<R> *>(p.out);
-  const cx<R> *tw = P.tw_lds ? twl : reinterpret_cast<const cx<R> *>(p.twiddle);
 
   int s0 = 0, s1 = p.nstage;
   if (P.fuse_in) {
@@ -324,30 +332,36 @@ __global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
   __syncthreads();
   if (P.fuse_out) s1 = p.nstage - 1;
 
-  for (int s = s0; s < s1; s++) {
-    switch (p.radix[s]) {
-      case 2: stage<R, INV, 2, CAP, NTH>(buf, P, s, tw); break;
-      case 3: if constexpr (!POW2) stage<R, INV, 3, CAP, NTH>(buf, P, s, tw); break;
-      case 4: stage<R, INV, 4, CAP, NTH>(buf, P, s, tw); break;
-      case 5: if constexpr (!POW2) stage<R, INV, 5, CAP, NTH>(buf, P, s, tw); break;
-      case 7: if constexpr (!POW2) stage<R, INV, 7, CAP, NTH>(buf, P, s, tw); break;
-      case 8: stage<R, INV, 8, CAP, NTH>(buf, P, s, tw); break;
-      case 11: if constexpr (!POW2) stage<R, INV, 11, CAP, NTH>(buf, P, s, tw); break;
-      case 13: if constexpr (!POW2) stage<R, INV, 13, CAP, NTH>(buf, P, s, tw); break;
-      case 16: stage<R, INV, 16, CAP, NTH>(buf, P, s, tw); break;
-      default: break;
+  // stages and the fused store, instantiated for LDS or global twiddles
+  auto run_stages = [&](auto tw) {
+    for (int s = s0; s < s1; s++) {
+      switch (p.radix[s]) {
+        case 2: stage<R, INV, 2, CAP, NTH>(buf, P, s, tw); break;
+        case 3: if constexpr (!POW2) stage<R, INV, 3, CAP, NTH>(buf, P, s, tw); break;
+        case 4: stage<R, INV, 4, CAP, NTH>(buf, P, s, tw); break;
+        case 5: if constexpr (!POW2) stage<R, INV, 5, CAP, NTH>(buf, P, s, tw); break;
+        case 7: if constexpr (!POW2) stage<R, INV, 7, CAP, NTH>(buf, P, s, tw); break;
+        case 8: stage<R, INV, 8, CAP, NTH>(buf, P, s, tw); break;
+        case 11: if constexpr (!POW2) stage<R, INV, 11, CAP, NTH>(buf, P, s, tw); break;
+        case 13: if constexpr (!POW2) stage<R, INV, 13, CAP, NTH>(buf, P, s, tw); break;
+        case 16: stage<R, INV, 16, CAP, NTH>(buf, P, s, tw); break;
+        default: break;
+      }
     }
-  }
 
-  if (P.fuse_out) {
-    switch (p.radix[p.nstage - 1]) {
-      case 2: if constexpr ((CAP / NTH) % 2 == 0) last_stage_store<R, INV, 2, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-      case 4: if constexpr ((CAP / NTH) % 4 == 0) last_stage_store<R, INV, 4, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-      case 8: if constexpr ((CAP / NTH) % 8 == 0) last_stage_store<R, INV, 8, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-      case 16: if constexpr ((CAP / NTH) % 16 == 0) last_stage_store<R, INV, 16, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-      default: break;
+    if (P.fuse_out) {
+      switch (p.radix[p.nstage - 1]) {
+        case 2: if constexpr ((CAP / NTH) % 2 == 0) last_stage_store<R, INV, 2, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+        case 4: if constexpr ((CAP / NTH) % 4 == 0) last_stage_store<R, INV, 4, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+        case 8: if constexpr ((CAP / NTH) % 8 == 0) last_stage_store<R, INV, 8, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+        case 16: if constexpr ((CAP / NTH) % 16 == 0) last_stage_store<R, INV, 16, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
+        default: break;
+      }
     }
-  } else {
+  };
+  if (P.tw_lds) run_stages((const lds_cx<R> *)twl);
+  else run_stages(reinterpret_cast<const cx<R> *>(p.twiddle));
+  if (!P.fuse_out) {
     i64 a[V];
     int sl[V];
     bool ok[V];
