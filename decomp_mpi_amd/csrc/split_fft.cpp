@@ -204,6 +204,60 @@ static void program_release(dm_fft_program &p) {
   p.passes.clear();
 }
 
+// A dense intermediate layout: axes in `order` (fastest first), the slowest
+// axis split into q equal blocks, one per rank slot (q the largest divisor
+// of its length not above P).  False when a rank's block would not fit the
+// region of `region` elements per slot.
+static bool dense_layout(const std::vector<int> &L, const std::vector<int> &order, int P, i64 region,
+                         dm::HostLayout *out) {
+  const int nd = (int)L.size();
+  const int slow = order.back();
+  int q = 1;
+  for (int c = std::min(P, L[slow]); c >= 1; c--)
+    if (L[slow] % c == 0) {
+      q = c;
+      break;
+    }
+  i64 total = 1;
+  for (int l : L) total *= l;
+  const i64 per = total / q;
+  if (per > region) return false;
+  dm::HostLayout h;
+  h.ndim = nd;
+  h.L = L;
+  h.proc_used = q;
+  h.rank_part.assign(nd, std::vector<int>());
+  h.off.assign(nd, std::vector<int>());
+  h.m.assign(nd, 0);
+  const int blk = L[slow] / q;
+  i64 stride = 1;
+  for (int k = 0; k < nd; k++) {
+    int d = order[k];
+    h.m[d] = stride;
+    stride *= (d == slow) ? blk : L[d];
+  }
+  for (int d = 0; d < nd; d++) {
+    h.rank_part[d].assign(L[d], 0);
+    h.off[d].resize(L[d]);
+    for (int g = 0; g < L[d]; g++) {
+      if (d == slow) {
+        h.rank_part[d][g] = g / blk;
+        h.off[d][g] = g % blk;
+      } else {
+        h.off[d][g] = g;
+      }
+    }
+  }
+  h.strides.clear();
+  for (int r = 0; r < q; r++) h.strides.insert(h.strides.end(), h.m.begin(), h.m.end());
+  h.uniform = true;
+  h.slot_extent.assign(q, per);
+  h.fastest = order[0];
+  h.offsets_ok = true;
+  *out = h;
+  return true;
+}
+
 // Build one direction's passes.  Forward: dat (start) -> W0 -> ... -> W0
 // (output); backward: W0 -> ... -> W0 -> dat.  The intermediate passes work
 // in place: every tile reads all of its points before writing them, tiles
@@ -225,16 +279,63 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
     if (d != f) rest.push_back(d);
   std::sort(rest.begin(), rest.end(), [&](int a, int b) { return O.h.m[a] < O.h.m[b]; });
   order.insert(order.end(), rest.begin(), rest.end());
+  const std::vector<int> forder = order;  // the forward pass order
   if (inverse) std::reverse(order.begin(), order.end());
   int n = (int)order.size();
+  // Intermediate layouts.  Dense: after the pass over axis order[i] the data
+  // are stored with the next pass's axis fastest and this pass's axis next,
+  // so every pass reads whole contiguous lines and writes rows of T
+  // contiguous elements at the stride of one line; the intermediates
+  // alternate between the workspace halves.  Otherwise (or when a dense
+  // block does not fit a slot) they stay in the output layout, in place.
+  static int dense_env = -1;
+  if (dense_env < 0) {
+    const char *e = getenv("DM_DENSE_INTERMEDIATES");
+    dense_env = e ? atoi(e) : 1;
+  }
+  // mids[k]: the layout after forward pass k.  The backward program runs the
+  // forward's passes in reverse, through the same layouts.
+  std::vector<dm::Side> mids;
+  bool dense = dense_env != 0 && n >= 2;
+  if (dense) {
+    const i64 region = (i64)plan->extent;
+    for (int i = 0; i + 1 < n && dense; i++) {
+      int cur = forder[i], next = forder[i + 1];
+      std::vector<int> ord = {next, cur};
+      std::vector<int> rest;
+      for (int d = 0; d < nd; d++)
+        if (d != cur && d != next) rest.push_back(d);
+      std::sort(rest.begin(), rest.end(), [&](int a, int b) { return O.h.m[a] < O.h.m[b]; });
+      ord.insert(ord.end(), rest.begin(), rest.end());
+      dm::HostLayout h;
+      if (!dense_layout(x->L, ord, plan->comm->world->nranks, region, &h)) dense = false;
+      else mids.push_back(dm::make_side(h, O.slot));
+    }
+  }
   dm::TableBuilder tb;
   std::vector<dm::PendingPass> pp(n);
   void *prev = nullptr;
   for (int i = 0; i < n; i++) {
     void *bin, *bout;
     const dm::Side *si, *so;
-    (void)w1;  // the workspace's second half stays free (the reference's 2 x extent contract)
-    if (!inverse) {
+    if (dense) {
+      // intermediate i (after pass i) in W1 when n - 2 - i is even, else W0,
+      // so that the last pass writes W0 (forward) from the other half
+      auto mid_buf = [&](int k) { return ((n - 2 - k) % 2 == 0) ? w1 : w0; };
+      if (!inverse) {
+        bin = (i == 0) ? dat : prev;
+        si = (i == 0) ? &S : &mids[i - 1];
+        bout = (i == n - 1) ? w0 : mid_buf(i);
+        so = (i == n - 1) ? &O : &mids[i];
+      } else {
+        // backward pass i undoes forward pass n-1-i: from the layout after
+        // it to the layout before it
+        bin = (i == 0) ? w0 : prev;
+        si = (i == 0) ? &O : &mids[n - 1 - i];
+        bout = (i == n - 1) ? dat : ((i % 2 == 0) ? w1 : w0);
+        so = (i == n - 1) ? &S : &mids[n - 2 - i];
+      }
+    } else if (!inverse) {
       bout = w0;
       bin = (i == 0) ? dat : prev;
       si = (i == 0) ? &S : &O;
