@@ -449,6 +449,15 @@ void launch_radices(const Pass &P, size_t lds, hipStream_t s) {
   bool pow2 = true;
   for (int i = 0; i < P.p.nstage; i++) pow2 = pow2 && (P.p.radix[i] & (P.p.radix[i] - 1)) == 0;
   bool small = P.p.T * P.p.N <= CAP / 2;
+  if constexpr (sizeof(R) == 8) {
+    // wide complex-double tile: 16 lines (256-byte column rows) on 1024
+    // threads, for column walks at a large line stride
+    if (P.p.T * P.p.N > CAP) {
+      if (pow2) launch_layouts<R, INV, 2 * CAP, 2 * NTH_BIG, true>(P, lds, s);
+      else launch_layouts<R, INV, 2 * CAP, 2 * NTH_BIG, false>(P, lds, s);
+      return;
+    }
+  }
   if (small) {
     if (pow2) launch_layouts<R, INV, CAP / 2, NTH_BIG / 2, true>(P, lds, s);
     else launch_layouts<R, INV, CAP / 2, NTH_BIG / 2, false>(P, lds, s);
@@ -463,13 +472,14 @@ void launch_radices(const Pass &P, size_t lds, hipStream_t s) {
 // Points per tile the planner should use.  Column walks need the big tile
 // (T * element = 128-byte rows); pure row walks use the half tile (twice the
 // workgroups per CU).  DM_FFT_TILE=big|small forces one size.
-int fft_tile_capacity(int dbl, int column) {
+int fft_tile_capacity(int dbl, int column, int wide) {
   static int force = -2;
   if (force == -2) {
     const char *e = getenv("DM_FFT_TILE");
     force = e == nullptr ? -1 : (e[0] == 'b' ? 0 : 1);
   }
   int cap = dbl ? CAP_D : CAP_F;
+  if (dbl && column && wide && force < 0) return 2 * cap;
   bool small = force >= 0 ? force == 1 : !column;
   return small ? cap / 2 : cap;
 }

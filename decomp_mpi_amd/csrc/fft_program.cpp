@@ -130,7 +130,14 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   L.in_row = in_row ? 1 : 0;
   L.out_row = out_row ? 1 : 0;
   if (u >= 0) {
-    int tcap = fft_tile_capacity(dbl, !(in_row && out_row));
+    // column walk at a line stride of 64 KiB or more: wide rows pay for the
+    // lower occupancy of the wide tile
+    const size_t es = dbl ? 16 : 8;
+    bool wide = (!in_row && (size_t)I.h.m[t] * es >= (64u << 10)) ||
+                (!out_row && (size_t)U.h.m[t] * es >= (64u << 10));
+    static int nowide = -1;
+    if (nowide < 0) nowide = getenv("DM_NO_WIDE") != nullptr;
+    int tcap = fft_tile_capacity(dbl, !(in_row && out_row), wide && !nowide);
     L.T = std::max(1, std::min(std::min(tcap / L.N, I.h.L[u]), 512));
     L.tiles_u = (I.h.L[u] + L.T - 1) / L.T;
   } else {

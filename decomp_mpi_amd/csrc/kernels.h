@@ -88,8 +88,11 @@ struct FftLaunch {
 };
 // nblocks == 0: the launch covers every tile of the pass.
 void launch_fft_pass(const FftLaunch &p, void *stream);
-// Largest points per tile the FFT pass kernel holds for an element size.
-int fft_tile_capacity(int dbl, int column);
+// Largest points per tile the FFT pass kernel holds for an element size:
+// row walks a half tile, column walks the full one, and complex-double
+// column walks at a large line stride ("wide") a double one (16 lines of
+// 512 points, 256-byte rows).
+int fft_tile_capacity(int dbl, int column, int wide = 0);
 
 // ---- K3: batched local FFT pass (many transforms, one axis of each) --------
 // A tile holds nl (<= T) lines of N points of one transform: point x of line
