@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "reshape.h"
@@ -85,6 +86,13 @@ struct Buffer {
 };
 // Resolve a pointer into slot 0 of a registered buffer.
 bool resolve(const void *p, Buffer *buf, size_t *offset);
+// Resolve a pointer into any slot: the buffer, the slot's rank, the offset.
+bool resolve_rank(const void *p, Buffer *buf, int *rank, size_t *offset);
+// Inside a dm_spmd_run session (spmd.cpp): the calling thread's rank (-1
+// outside), and the collective rendezvous -- all ranks wait, rank 0 runs fn,
+// all ranks wait again.
+int spmd_rank();
+void spmd_collective(const std::function<void()> &fn);
 
 // ---- layouts (layout.cpp)
 // A decomposition flattened into per-axis lookup tables over global

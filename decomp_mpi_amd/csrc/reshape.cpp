@@ -356,8 +356,20 @@ bool enqueue_reshape(dm_comm_t comm, ReshapePlan plan, const void *src, void *ds
 }
 }  // namespace dm
 
-// reshape.c:461-492
+static void run_reshape(MPI_Comm comm, ReshapePlan plan, void *src, void *dest);
+
+// reshape.c:461-492.  Collective: inside a dm_spmd_run session every rank
+// calls it with pointers into its own slots; rank 0's call (whose slot
+// pointers are the buffers' slot-0 pointers) moves every rank's share.
 extern "C" void runReshapePlan(MPI_Comm comm, ReshapePlan plan, void *src, void *dest) {
+  if (dm::spmd_rank() >= 0) {
+    dm::spmd_collective([&] { run_reshape(comm, plan, src, dest); });
+    return;
+  }
+  run_reshape(comm, plan, src, dest);
+}
+
+static void run_reshape(MPI_Comm comm, ReshapePlan plan, void *src, void *dest) {
   if (plan == nullptr) {
     dm::fail(comm, "runReshapePlan: NULL plan");
     return;

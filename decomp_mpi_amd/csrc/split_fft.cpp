@@ -728,7 +728,19 @@ static void run_program(SplitPlan plan, dm_fft_program &prog, int t_first, int t
 // work.  Backward: the exact (unnormalised) inverse, output layout in work ->
 // start layout in dat (the reference's backward path runs A_to_start before
 // its stage-1 inverse and builds that inverse only under MAKE_REVERSE).
+static void run_split(SplitPlan plan, int direction);
+
+// Collective like runReshapePlan: in a dm_spmd_run session rank 0's plan
+// (built on slot-0 pointers) runs for every rank.
 extern "C" void runSplitPlan(SplitPlan plan, int direction) {
+  if (dm::spmd_rank() >= 0) {
+    dm::spmd_collective([&] { run_split(plan, direction); });
+    return;
+  }
+  run_split(plan, direction);
+}
+
+static void run_split(SplitPlan plan, int direction) {
   if (plan == nullptr) {
     dm::fail(nullptr, "runSplitPlan: NULL plan");
     return;

@@ -220,6 +220,20 @@ bool resolve(const void *p, Buffer *buf, size_t *offset) {
   *offset = a - it->first;
   return true;
 }
+
+bool resolve_rank(const void *p, Buffer *buf, int *rank, size_t *offset) {
+  std::lock_guard<std::mutex> lk(g_buf_mutex);
+  uintptr_t a = (uintptr_t)p;
+  auto it = g_buffers.upper_bound(a);
+  if (it == g_buffers.begin()) return false;
+  --it;
+  const Buffer &b = it->second;
+  if (a < it->first || a >= it->first + b.slot * (size_t)b.nranks) return false;
+  *buf = b;
+  *rank = (int)((a - it->first) / b.slot);
+  *offset = (a - it->first) % b.slot;
+  return true;
+}
 }  // namespace dm
 
 extern "C" void *dm_malloc(MPI_Comm comm, size_t bytes_per_rank) {
