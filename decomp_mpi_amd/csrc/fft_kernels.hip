@@ -65,7 +65,8 @@ struct LineAddr<true> {
       if (d != p.t) a += abs[d][g[d]];
     base[l] = a;
   }
-  __device__ __forceinline__ i64 at(const FftLaunch &, const FftSide &, const i64 *absT, int l, int x) const {
+  __device__ __forceinline__ i64 at(const FftLaunch &, const FftSide &s, const i64 *absT, int l, int x) const {
+    if (s.aff) return base[l] + (i64)x * s.aff;  // affine along t: no table load
     return base[l] + absT[x];
   }
 };

@@ -164,6 +164,15 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     fs.uniform = s.h.uniform ? 1 : 0;
     fs.proc_used = s.h.proc_used;
     fs.slot = s.slot;
+    // offsets along t affine (x * m_t: axis t local to every rank, in order)
+    fs.aff = 0;
+    if (s.h.uniform && s.h.m[t] > 0) {
+      bool affine = true;
+      for (int x = 0; x < s.h.L[t] && affine; x++) affine = s.h.rank_part[t][x] == 0 && s.h.off[t][x] == x;
+      static int noaff = -1;
+      if (noaff < 0) noaff = getenv("DM_NO_AFFINE") != nullptr;
+      if (affine && !noaff) fs.aff = s.h.m[t];
+    }
     if (s.h.uniform) return;
     for (int d = 0; d < nd; d++) {
       ps.rank[d] = tb.put(s.h.rank_part[d].data(), s.h.L[d] * sizeof(int));

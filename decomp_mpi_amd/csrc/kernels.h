@@ -50,6 +50,7 @@ struct FftSide {
   int uniform;
   int proc_used;
   i64 slot;                    // elements per rank slot
+  i64 aff;                     // uniform side whose offset along t is x * aff (0: use the table)
   const int *rank[DM_MAX_DIMS];
   const int *off[DM_MAX_DIMS];
   const i64 *strides;          // [proc_used * ndim]
