@@ -17,6 +17,9 @@ namespace {
 
 using namespace fftdev;
 
+template <typename R>
+using gptr = __attribute__((address_space(1))) cx<R> *;  // global memory
+
 constexpr int BCAP_D = 4096;
 constexpr int BCAP_F = 4096;  // 8 points per thread in both precisions
 constexpr int BNTH = 512;
@@ -36,7 +39,9 @@ __global__ __launch_bounds__(NTH) void batch_pass_kernel(BatchLaunch p) {
     const int tot = T * N;
     const int ls = lds_line(N);
     const long long lstride = d.row ? (long long)N : 1LL;
-    cx<R> *data = reinterpret_cast<cx<R> *>(p.data[d.xform]) + d.base;
+    // the transform's pointer comes from memory, so its address space is
+    // stated: through a generic pointer every access would be a flat one
+    gptr<R> data = (gptr<R>)(p.data[d.xform]) + d.base;
     // load: addresses, then data, then LDS (branch-free; absent elements
     // read element 0 of the tile and are dropped)
     long long a[V];
@@ -60,7 +65,7 @@ __global__ __launch_bounds__(NTH) void batch_pass_kernel(BatchLaunch p) {
     }
     cx<R> v[V];
 #pragma unroll
-    for (int k = 0; k < V; k++) v[k] = data[a[k]];
+    for (int k = 0; k < V; k++) v[k] = {data[a[k]].x, data[a[k]].y};
 #pragma unroll
     for (int k = 0; k < V; k++)
       if (sl[k] >= 0) buf[sl[k]] = ok[k] ? v[k] : cx<R>{0, 0};
@@ -83,7 +88,11 @@ __global__ __launch_bounds__(NTH) void batch_pass_kernel(BatchLaunch p) {
     // store: same element walk, straight from LDS
 #pragma unroll
     for (int k = 0; k < V; k++)
-      if (ok[k]) data[a[k]] = buf[sl[k]];
+      if (ok[k]) {
+        const cx<R> w = buf[sl[k]];
+        data[a[k]].x = w.x;
+        data[a[k]].y = w.y;
+      }
     __syncthreads();  // the next tile reuses the LDS tile
   }
 }
