@@ -24,6 +24,93 @@ constexpr int BCAP_D = 4096;
 constexpr int BCAP_F = 4096;  // 8 points per thread in both precisions
 constexpr int BNTH = 512;
 
+// Butterfly bi of a stage with nr butterflies per line: row walks put
+// consecutive butterflies of a line on consecutive threads, column walks
+// consecutive lines (so that each butterfly input is one coalesced access).
+__device__ __forceinline__ void bfly(int bi, int row, int nr, unsigned long long m_nr, int T,
+                                     unsigned long long m_t, int &l, int &j) {
+  if (row) {
+    l = fdiv(bi, m_nr);
+    j = bi - l * nr;
+  } else {
+    j = fdiv(bi, m_t);
+    l = bi - j * T;
+  }
+}
+
+// First stage on the loaded values: whole butterflies per thread (points
+// j + nr r of line l), no twiddles, outputs to their Stockham places.
+template <typename R, bool INV, int RADIX, int CAP, int NTH>
+__device__ __forceinline__ void load_first(cx<R> *buf, const BatchSched &S, const BatchTile &d,
+                                           gptr<R> data, long long lstride, int ls) {
+  constexpr int K = (CAP / NTH) / RADIX;
+  const StageInfo &st = S.st[0];
+  const int nr = st.nr, nb = S.T * nr;
+  long long a[K][RADIX];
+  int lk[K], jk[K];
+  bool ok[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int bi = threadIdx.x + q * NTH;
+    bfly(min(bi, nb - 1), d.row, nr, st.m_nr, S.T, S.m_t, lk[q], jk[q]);
+    ok[q] = bi < nb && lk[q] < d.nl;
+#pragma unroll
+    for (int r = 0; r < RADIX; r++)
+      a[q][r] = ok[q] ? (long long)lk[q] * lstride + (long long)(jk[q] + r * nr) * d.ps : 0;
+  }
+  cx<R> v[K][RADIX];
+#pragma unroll
+  for (int q = 0; q < K; q++)
+#pragma unroll
+    for (int r = 0; r < RADIX; r++) v[q][r] = {data[a[q][r]].x, data[a[q][r]].y};
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    if (!ok[q])
+#pragma unroll
+      for (int r = 0; r < RADIX; r++) v[q][r] = {0, 0};
+    dft<R, INV, RADIX>(v[q]);
+    if ((int)threadIdx.x + q * NTH < nb) {
+      const int base = lk[q] * ls, idx = jk[q] * RADIX;
+#pragma unroll
+      for (int r = 0; r < RADIX; r++) buf[base + lds_pad(idx + r)] = v[q][r];
+    }
+  }
+}
+
+// Last stage straight to memory: butterfly inputs from LDS, twiddles
+// (stage-ordered: Ns = nr in the last stage), transform, store.
+template <typename R, bool INV, int RADIX, int CAP, int NTH>
+__device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S, const BatchTile &d,
+                                           gptr<R> data, long long lstride, int ls, const cx<R> *tw) {
+  constexpr int K = (CAP / NTH) / RADIX;
+  const StageInfo &st = S.st[S.nstage - 1];
+  const int nr = st.nr, nb = S.T * nr;
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int bi = threadIdx.x + q * NTH;
+    int l, j;
+    bfly(min(bi, nb - 1), d.row, nr, st.m_nr, S.T, S.m_t, l, j);
+    const bool ok = bi < nb && l < d.nl;
+    cx<R> v[RADIX];
+#pragma unroll
+    for (int r = 0; r < RADIX; r++) v[r] = buf[l * ls + lds_pad(j + r * nr)];
+#pragma unroll
+    for (int r = 1; r < RADIX; r++) {
+      cx<R> w = tw[(nr - 1) + (r - 1) * nr + j];
+      if (INV) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+    dft<R, INV, RADIX>(v);
+    if (ok)
+#pragma unroll
+      for (int r = 0; r < RADIX; r++) {
+        const long long at = (long long)l * lstride + (long long)(j + r * nr) * d.ps;
+        data[at].x = v[r].x;
+        data[at].y = v[r].y;
+      }
+  }
+}
+
 // PRIMES: schedules with radix 5, 7, 11 or 13 (their butterflies need more
 // registers, so launches without them use a lighter kernel).
 template <typename R, bool INV, int CAP, int NTH, bool PRIMES>
@@ -42,36 +129,48 @@ __global__ __launch_bounds__(NTH) void batch_pass_kernel(BatchLaunch p) {
     // the transform's pointer comes from memory, so its address space is
     // stated: through a generic pointer every access would be a flat one
     gptr<R> data = (gptr<R>)(p.data[d.xform]) + d.base;
-    // load: addresses, then data, then LDS (branch-free; absent elements
-    // read element 0 of the tile and are dropped)
-    long long a[V];
-    int sl[V];
-    bool ok[V];
-#pragma unroll
-    for (int k = 0; k < V; k++) {
-      int e = threadIdx.x + k * NTH;
-      int ec = min(e, tot - 1);
-      int l, x;
-      if (d.row) {
-        l = fdiv(ec, S.m_n);
-        x = ec - l * N;
-      } else {
-        x = fdiv(ec, S.m_t);
-        l = ec - x * T;
-      }
-      ok[k] = e < tot && l < nl;
-      sl[k] = e < tot ? lds_index(l, x, ls) : -1;
-      a[k] = ok[k] ? (long long)l * lstride + (long long)x * d.ps : 0;
-    }
-    cx<R> v[V];
-#pragma unroll
-    for (int k = 0; k < V; k++) v[k] = {data[a[k]].x, data[a[k]].y};
-#pragma unroll
-    for (int k = 0; k < V; k++)
-      if (sl[k] >= 0) buf[sl[k]] = ok[k] ? v[k] : cx<R>{0, 0};
-    __syncthreads();
     const cx<R> *tw = twall + S.tw_off;
-    for (int s = 0; s < S.nstage; s++) {
+    // Fused first / last stages: a power-of-two radix dividing the points per
+    // thread (whole butterflies per thread); the last one needs two stages.
+    const int r0 = S.radix[0], rl = S.radix[S.nstage - 1];
+    const bool fuse_in = (r0 == 2 || r0 == 4 || r0 == 8) && V % r0 == 0;
+    const bool fuse_out = S.nstage >= 2 && (rl == 2 || rl == 4 || rl == 8) && V % rl == 0;
+    if (fuse_in) {
+      if (r0 == 8) load_first<R, INV, 8, CAP, NTH>(buf, S, d, data, lstride, ls);
+      else if (r0 == 4) load_first<R, INV, 4, CAP, NTH>(buf, S, d, data, lstride, ls);
+      else load_first<R, INV, 2, CAP, NTH>(buf, S, d, data, lstride, ls);
+    } else {
+      // load: addresses, then data, then LDS (branch-free; absent elements
+      // read element 0 of the tile and are dropped)
+      long long a[V];
+      int sl[V];
+      bool ok[V];
+#pragma unroll
+      for (int k = 0; k < V; k++) {
+        int e = threadIdx.x + k * NTH;
+        int ec = min(e, tot - 1);
+        int l, x;
+        if (d.row) {
+          l = fdiv(ec, S.m_n);
+          x = ec - l * N;
+        } else {
+          x = fdiv(ec, S.m_t);
+          l = ec - x * T;
+        }
+        ok[k] = e < tot && l < nl;
+        sl[k] = e < tot ? lds_index(l, x, ls) : -1;
+        a[k] = ok[k] ? (long long)l * lstride + (long long)x * d.ps : 0;
+      }
+      cx<R> v[V];
+#pragma unroll
+      for (int k = 0; k < V; k++) v[k] = {data[a[k]].x, data[a[k]].y};
+#pragma unroll
+      for (int k = 0; k < V; k++)
+        if (sl[k] >= 0) buf[sl[k]] = ok[k] ? v[k] : cx<R>{0, 0};
+    }
+    __syncthreads();
+    const int s0 = fuse_in ? 1 : 0, s1 = fuse_out ? S.nstage - 1 : S.nstage;
+    for (int s = s0; s < s1; s++) {
       switch (S.radix[s]) {
         case 2: stage_lds<R, INV, 2, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
         case 3: stage_lds<R, INV, 3, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
@@ -85,14 +184,32 @@ __global__ __launch_bounds__(NTH) void batch_pass_kernel(BatchLaunch p) {
         default: break;
       }
     }
-    // store: same element walk, straight from LDS
+    if (fuse_out) {
+      if (rl == 8) last_store<R, INV, 8, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
+      else if (rl == 4) last_store<R, INV, 4, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
+      else last_store<R, INV, 2, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
+    } else {
+      // store: same element walk as the load, straight from LDS
 #pragma unroll
-    for (int k = 0; k < V; k++)
-      if (ok[k]) {
-        const cx<R> w = buf[sl[k]];
-        data[a[k]].x = w.x;
-        data[a[k]].y = w.y;
+      for (int k = 0; k < V; k++) {
+        int e = threadIdx.x + k * NTH;
+        int ec = min(e, tot - 1);
+        int l, x;
+        if (d.row) {
+          l = fdiv(ec, S.m_n);
+          x = ec - l * N;
+        } else {
+          x = fdiv(ec, S.m_t);
+          l = ec - x * T;
+        }
+        if (e < tot && l < nl) {
+          const cx<R> w = buf[lds_index(l, x, ls)];
+          const long long at = (long long)l * lstride + (long long)x * d.ps;
+          data[at].x = w.x;
+          data[at].y = w.y;
+        }
       }
+    }
     __syncthreads();  // the next tile reuses the LDS tile
   }
 }
