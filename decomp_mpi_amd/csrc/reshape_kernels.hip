@@ -111,15 +111,39 @@ __global__ __launch_bounds__(NTHREADS) void reshape_tile2d(ReshapeLaunch p) {
 
   // load: consecutive threads walk axis b
   i64 s[NB][RPT];
-#pragma unroll
-  for (int n = 0; n < NB; n++)
+  if constexpr (US) {
+    // uniform source: the tile axes' table entries are loaded once (one per
+    // column, one per row) and shared by the NB slices
+    const int gbc = min(b0 + tx, Lb - 1);
+    const int rb = p.src.rank[p.b][gbc];
+    const i64 ab = p.src.addr[p.b][gbc];
+    int ra[RPT];
+    i64 aa[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-      int ga = a0 + ty + k * ROWS, gb = b0 + tx;
-      bool in = live[n] && ga < La && gb < Lb;
-      s[n][k] = locate<US>(p.src, p, qs[n], p.a, in ? ga : 0, p.b, in ? gb : 0, p.src_slot);
-      if (!in) s[n][k] = -1;
+      const int gac = min(a0 + ty + k * ROWS, La - 1);
+      ra[k] = p.src.rank[p.a][gac];
+      aa[k] = p.src.addr[p.a][gac];
     }
+#pragma unroll
+    for (int n = 0; n < NB; n++)
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        const bool in = live[n] && a0 + ty + k * ROWS < La && b0 + tx < Lb;
+        const int R = qs[n].rank + ra[k] + rb;
+        s[n][k] = (in && R < p.src.proc_used) ? (i64)R * p.src_slot + qs[n].addr + aa[k] + ab : -1;
+      }
+  } else {
+#pragma unroll
+    for (int n = 0; n < NB; n++)
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        int ga = a0 + ty + k * ROWS, gb = b0 + tx;
+        bool in = live[n] && ga < La && gb < Lb;
+        s[n][k] = locate<US>(p.src, p, qs[n], p.a, in ? ga : 0, p.b, in ? gb : 0, p.src_slot);
+        if (!in) s[n][k] = -1;
+      }
+  }
   E v[NB][RPT];
 #pragma unroll
   for (int n = 0; n < NB; n++)
@@ -135,15 +159,37 @@ __global__ __launch_bounds__(NTHREADS) void reshape_tile2d(ReshapeLaunch p) {
   __syncthreads();
   // store: consecutive threads walk axis a
   i64 d[NB][RPT];
-#pragma unroll
-  for (int n = 0; n < NB; n++)
+  if constexpr (UD) {
+    const int gac = min(a0 + tx, La - 1);
+    const int ra = p.dst.rank[p.a][gac];
+    const i64 aa = p.dst.addr[p.a][gac];
+    int rb[RPT];
+    i64 ab[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-      int ga = a0 + tx, gb = b0 + ty + k * ROWS;
-      bool in = live[n] && ga < La && gb < Lb;
-      d[n][k] = locate<UD>(p.dst, p, qd[n], p.a, in ? ga : 0, p.b, in ? gb : 0, p.dst_slot);
-      if (!in || !have[n][tx][ty + k * ROWS]) d[n][k] = -1;
+      const int gbc = min(b0 + ty + k * ROWS, Lb - 1);
+      rb[k] = p.dst.rank[p.b][gbc];
+      ab[k] = p.dst.addr[p.b][gbc];
     }
+#pragma unroll
+    for (int n = 0; n < NB; n++)
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        const bool in = live[n] && a0 + tx < La && b0 + ty + k * ROWS < Lb && have[n][tx][ty + k * ROWS];
+        const int R = qd[n].rank + ra + rb[k];
+        d[n][k] = (in && R < p.dst.proc_used) ? (i64)R * p.dst_slot + qd[n].addr + aa + ab[k] : -1;
+      }
+  } else {
+#pragma unroll
+    for (int n = 0; n < NB; n++)
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        int ga = a0 + tx, gb = b0 + ty + k * ROWS;
+        bool in = live[n] && ga < La && gb < Lb;
+        d[n][k] = locate<UD>(p.dst, p, qd[n], p.a, in ? ga : 0, p.b, in ? gb : 0, p.dst_slot);
+        if (!in || !have[n][tx][ty + k * ROWS]) d[n][k] = -1;
+      }
+  }
 #pragma unroll
   for (int n = 0; n < NB; n++)
 #pragma unroll
