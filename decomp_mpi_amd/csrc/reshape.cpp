@@ -169,6 +169,30 @@ static void extend_by_words(dm::HostLayout &h, int words) {
   h.ndim = nd + 1;
 }
 
+// Relabel the global coordinates of axis d on both sides so that they run in
+// `key`'s memory order (owner rank, then local offset).  A bijection applied
+// to both tables keeps the pairing by global coordinate, and a tile over the
+// relabelled axis then starts where one of `key`'s runs starts: with a
+// rotated or interleaved side, global tiles would straddle its run
+// boundaries and every row would touch partial cache lines.
+static void relabel_axis(dm_reshape_program *pg, int d, const dm::HostLayout &key) {
+  const int n = key.L[d];
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  const std::vector<int> &kr = key.rank_part[d], &ko = key.off[d];
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int x, int y) { return kr[x] != kr[y] ? kr[x] < kr[y] : ko[x] < ko[y]; });
+  for (dm::HostLayout *h : {&pg->src, &pg->dst}) {
+    std::vector<int> r(n), o(n);
+    for (int i = 0; i < n; i++) {
+      r[i] = h->rank_part[d][perm[i]];
+      o[i] = h->off[d][perm[i]];
+    }
+    h->rank_part[d].swap(r);
+    h->off[d].swap(o);
+  }
+}
+
 extern "C" ReshapePlan makeReshapePlan(MPI_Comm comm, MPI_Datatype base, Decomp src, Decomp dest) {
   if (src == nullptr || dest == nullptr || base == nullptr) {
     dm::fail(comm, "makeReshapePlan: NULL argument");
@@ -250,6 +274,11 @@ extern "C" ReshapePlan makeReshapePlan(MPI_Comm comm, MPI_Datatype base, Decomp 
     };
     pg->a = pick(pg->dst);
     pg->b = pick(pg->src);
+  }
+  // writes run along a, reads along b
+  if (getenv("DM_NO_RELABEL") == nullptr) {
+    relabel_axis(pg, pg->a, pg->dst);
+    if (pg->b != pg->a) relabel_axis(pg, pg->b, pg->src);
   }
   std::vector<int> rest;
   for (int d = 0; d < pg->dst.ndim; d++)
