@@ -256,5 +256,38 @@ __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T
   __syncthreads();
 }
 
+// Global data accesses of a pass: N elements at offsets a[k] from base,
+// optionally with the non-temporal hint (data streamed once per launch).
+// The hint is uniform across the launch, so the branch is taken once.
+template <int N, typename R, typename P>
+__device__ __forceinline__ void gather(cx<R> *v, P src, const i64 *a, bool nt) {
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  if (nt) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const v2 t = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(&src[a[k]]));
+      v[k] = {t.x, t.y};
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = src[a[k]];
+  }
+}
+
+// Stores element k when ok[k / PER] holds.
+template <int N, int PER, typename R, typename P>
+__device__ __forceinline__ void scatter(P dst, const i64 *a, const cx<R> *v, const bool *ok, bool nt) {
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  if (nt) {
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      if (ok[k / PER]) __builtin_nontemporal_store(v2{v[k].x, v[k].y}, reinterpret_cast<v2 *>(&dst[a[k]]));
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      if (ok[k / PER]) dst[a[k]] = v[k];
+  }
+}
+
 }  // namespace fftdev
 }  // namespace dm

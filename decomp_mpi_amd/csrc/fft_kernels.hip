@@ -191,10 +191,7 @@ __device__ __forceinline__ void load_first_stage(cx<R> *buf, const Pass &P, cons
     for (int r = 0; r < RADIX; r++) a[q][r] = lin.at(p, p.sin, inT, ls, jk[q] + r * nr);
   }
   cx<R> v[K][RADIX];
-#pragma unroll
-  for (int q = 0; q < K; q++)
-#pragma unroll
-    for (int r = 0; r < RADIX; r++) v[q][r] = in[a[q][r]];
+  gather<K * RADIX>(&v[0][0], in, &a[0][0], p.stream_in);
 #pragma unroll
   for (int q = 0; q < K; q++) {
     if (!ok[q])
@@ -244,11 +241,7 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
 #pragma unroll
     for (int r = 0; r < RADIX; r++) a[q][r] = lout.at(p, p.sout, outT, ls, j + r * nr);
   }
-#pragma unroll
-  for (int q = 0; q < K; q++)
-    if (ok[q])
-#pragma unroll
-      for (int r = 0; r < RADIX; r++) out[a[q][r]] = v[q][r];
+  scatter<K * RADIX, RADIX>(out, &a[0][0], &v[0][0], ok, p.stream_out);
 }
 
 // One tile per workgroup.  Loads and stores run in three branch-free
@@ -324,8 +317,7 @@ __global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
       a[k] = lin.at(p, p.sin, inT, ok[k] ? l : 0, x);
     }
     cx<R> v[V];
-#pragma unroll
-    for (int k = 0; k < V; k++) v[k] = in[a[k]];
+    gather<V>(v, in, a, p.stream_in);
 #pragma unroll
     for (int k = 0; k < V; k++)
       if (sl[k] >= 0) buf[sl[k]] = ok[k] ? v[k] : cx<R>{0, 0};
@@ -379,9 +371,7 @@ __global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
     cx<R> w[V];
 #pragma unroll
     for (int k = 0; k < V; k++) w[k] = buf[sl[k]];
-#pragma unroll
-    for (int k = 0; k < V; k++)
-      if (ok[k]) out[a[k]] = w[k];
+    scatter<V, 1>(out, a, w, ok, p.stream_out);
   }
 }
 

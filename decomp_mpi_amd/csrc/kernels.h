@@ -86,6 +86,7 @@ struct FftLaunch {
   void *out;
   long long tile_base;        // this launch runs tiles [tile_base, tile_base + nblocks)
   int pass_index;             // host bookkeeping: the pass this launch belongs to
+  int stream_in, stream_out;  // 1: non-temporal hint on the data loads / stores
 };
 // nblocks == 0: the launch covers every tile of the pass.
 void launch_fft_pass(const FftLaunch &p, void *stream);
