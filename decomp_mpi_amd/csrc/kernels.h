@@ -29,6 +29,7 @@ struct ReshapeLaunch {
   i64 src_slot;           // elements per rank slot
   i64 dst_slot;
   int elem_bytes;
+  int nontemporal;            // 1: non-temporal hint on the element accesses
 };
 void launch_reshape(const ReshapeLaunch &p, void *stream);
 

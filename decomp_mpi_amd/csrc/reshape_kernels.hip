@@ -26,7 +26,20 @@ template <> struct Elem<1> { typedef uint8_t T; };
 template <> struct Elem<2> { typedef uint16_t T; };
 template <> struct Elem<4> { typedef uint32_t T; };
 template <> struct Elem<8> { typedef uint64_t T; };
-template <> struct Elem<16> { typedef uint4 T; };
+template <> struct Elem<16> { typedef unsigned int T __attribute__((ext_vector_type(4))); };
+
+// Element accesses, with the non-temporal hint when NT (each element is
+// read once and written once per launch).
+template <bool NT, typename E>
+__device__ __forceinline__ E read_elem(const E *at) {
+  if constexpr (NT) return __builtin_nontemporal_load(at);
+  else return *at;
+}
+template <bool NT, typename E>
+__device__ __forceinline__ void write_elem(E *at, E v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, at);
+  else *at = v;
+}
 
 // Per-block partial sums of the batch axes for one layout.
 struct Partial {
@@ -81,7 +94,7 @@ __device__ __forceinline__ i64 locate(const DevLayout &Lt, const ReshapeLaunch &
 // to element 0 of the region when the element is absent), then the LDS (or
 // global) writes.  A load queue that completes in order would otherwise
 // stall each data load behind the next element's table loads.
-template <int BYTES, bool US, bool UD, int NB>
+template <int BYTES, bool US, bool UD, int NB, bool NT>
 __global__ __launch_bounds__(NTHREADS) void reshape_tile2d(ReshapeLaunch p) {
   typedef typename Elem<BYTES>::T E;
   constexpr int RPT = TILE / ROWS;
@@ -148,7 +161,7 @@ __global__ __launch_bounds__(NTHREADS) void reshape_tile2d(ReshapeLaunch p) {
 #pragma unroll
   for (int n = 0; n < NB; n++)
 #pragma unroll
-    for (int k = 0; k < RPT; k++) v[n][k] = src[s[n][k] >= 0 ? s[n][k] : 0];
+    for (int k = 0; k < RPT; k++) v[n][k] = read_elem<NT>(src + (s[n][k] >= 0 ? s[n][k] : 0));
 #pragma unroll
   for (int n = 0; n < NB; n++)
 #pragma unroll
@@ -194,11 +207,11 @@ __global__ __launch_bounds__(NTHREADS) void reshape_tile2d(ReshapeLaunch p) {
   for (int n = 0; n < NB; n++)
 #pragma unroll
     for (int k = 0; k < RPT; k++)
-      if (d[n][k] >= 0) dst[d[n][k]] = tile[n][tx][ty + k * ROWS];
+      if (d[n][k] >= 0) write_elem<NT>(dst + d[n][k], tile[n][tx][ty + k * ROWS]);
 }
 
 // 1D runs along axis a (same fastest axis on both sides), phased as above.
-template <int BYTES, bool US, bool UD>
+template <int BYTES, bool US, bool UD, bool NT>
 __global__ __launch_bounds__(NTHREADS) void reshape_run1d(ReshapeLaunch p) {
   typedef typename Elem<BYTES>::T E;
   constexpr int PER = 4;
@@ -224,20 +237,26 @@ __global__ __launch_bounds__(NTHREADS) void reshape_run1d(ReshapeLaunch p) {
   }
   E v[PER];
 #pragma unroll
-  for (int k = 0; k < PER; k++) v[k] = src[s[k] >= 0 ? s[k] : 0];
+  for (int k = 0; k < PER; k++) v[k] = read_elem<NT>(src + (s[k] >= 0 ? s[k] : 0));
 #pragma unroll
   for (int k = 0; k < PER; k++)
-    if (d[k] >= 0) dst[d[k]] = v[k];
+    if (d[k] >= 0) write_elem<NT>(dst + d[k], v[k]);
+}
+
+template <int BYTES, bool US, bool UD, bool NT>
+void launch_hinted(const ReshapeLaunch &p, hipStream_t s) {
+  if (p.a != p.b)
+    hipLaunchKernelGGL((reshape_tile2d<BYTES, US, UD, TILE_SLICES, NT>), dim3((unsigned)p.nblocks),
+                       dim3(NTHREADS), 0, s, p);
+  else
+    hipLaunchKernelGGL((reshape_run1d<BYTES, US, UD, NT>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0, s, p);
 }
 
 template <int BYTES, bool US, bool UD>
 void launch_typed(const ReshapeLaunch &p, hipStream_t s) {
   if (p.nblocks <= 0) return;
-  if (p.a != p.b)
-    hipLaunchKernelGGL((reshape_tile2d<BYTES, US, UD, TILE_SLICES>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0,
-                       s, p);
-  else
-    hipLaunchKernelGGL((reshape_run1d<BYTES, US, UD>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0, s, p);
+  if (p.nontemporal) launch_hinted<BYTES, US, UD, true>(p, s);
+  else launch_hinted<BYTES, US, UD, false>(p, s);
 }
 
 template <int BYTES>
