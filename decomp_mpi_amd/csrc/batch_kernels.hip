@@ -114,7 +114,7 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 // PRIMES: schedules with radix 5, 7, 11 or 13 (their butterflies need more
 // registers, so launches without them use a lighter kernel).
 template <typename R, bool INV, int CAP, int NTH, bool PRIMES>
-__global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 6 : 4) void batch_pass_kernel(BatchLaunch p) {
+__global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel(BatchLaunch p) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   constexpr int V = CAP / NTH;
