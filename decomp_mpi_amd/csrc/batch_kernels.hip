@@ -112,7 +112,10 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 }
 
 // PRIMES: schedules with radix 5, 7, 11 or 13 (their butterflies need more
-// registers, so launches without them use a lighter kernel).
+// registers, so launches without them use a lighter kernel).  Waves per
+// SIMD: complex float fits 64 VGPRs without spills, so 8 waves and four
+// 512-thread workgroups per CU keep more loads in flight (81 VGPRs allowed
+// only two); complex double spills below 104, so it stays at 4.
 template <typename R, bool INV, int CAP, int NTH, bool PRIMES>
 __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel(BatchLaunch p) {
   extern __shared__ char smem_raw[];
