@@ -107,7 +107,8 @@ _SIGS = {
     "freeDimDecomp": (None, [P]), "cloneDimDecomp": (P, [P]), "sizeDimDecomp": (I, [P]),
     # axis / decomp
     "makeAxisDecomp": (P, [P, P, I, I]), "freeAxisDecomp": (None, [P]), "isLocal": (I, [P, I]),
-    "makeIndexList": (P, [P, P]), "makeGlobalIndex": (P, [P]),
+    "makeIndexList": (P, [P, P]), "makeGlobalIndex": (P, [P]), "freeIndexList": (None, [P]),
+    "freeIndex": (None, [P]),
     "makeDecomp": (P, [I, P]), "makePackedDecomp": (P, [P, I, P]), "freeDecomp": (None, [P]),
     "extent": (I, [P]), "compatible": (I, [P, P]), "equivalent": (I, [P, P]), "myRank": (I, [P]),
     "isCoordLocal": (I, [P, I, I]), "areCoordsLocal": (I, [P, IP]), "getOffset": (I, [P, IP]),

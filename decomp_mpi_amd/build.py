@@ -22,6 +22,11 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              f"-I{INCLUDE}", f"-I{CSRC}"]
 
 
+# FFT pass kernel families: launcher name, real type, inverse, powers of two only
+FFT_FAMILIES = [(f"launch_{r[0]}_{d}_{k}", r, "true" if d == "inv" else "false", "true" if k == "p2" else "false")
+                for r in ("double", "float") for d in ("fwd", "inv") for k in ("p2", "mx")]
+
+
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -44,8 +49,18 @@ def build(verbose=False):
         o = os.path.join(OBJDIR, os.path.basename(s) + ".o")
         jobs.append(["g++", *HOST_FLAGS, "-c", "-o", o, s])
     for s in hip:
+        if os.path.basename(s) == "fft_inst.hip":
+            # one translation unit per kernel family (they build in parallel)
+            for name, real, inv, pow2 in FFT_FAMILIES:
+                o = os.path.join(OBJDIR, f"fft_inst_{name}.o")
+                defs = [f"-DDM_INST_NAME={name}", f"-DDM_INST_R={real}", f"-DDM_INST_INV={inv}",
+                        f"-DDM_INST_POW2={pow2}"]
+                jobs.append([f"{ROCM}/bin/hipcc", *HIP_FLAGS, *defs, "-c", "-o", o, s])
+            continue
         o = os.path.join(OBJDIR, os.path.basename(s) + ".o")
         jobs.append([f"{ROCM}/bin/hipcc", *HIP_FLAGS, "-c", "-o", o, s])
+    # the heaviest (complex-double) families first
+    jobs.sort(key=lambda j: 0 if any("DM_INST_R=double" in a for a in j) else 1)
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     objs = [j[j.index("-o") + 1] for j in jobs]

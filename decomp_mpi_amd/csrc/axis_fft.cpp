@@ -107,7 +107,8 @@ extern "C" int dm_run_axis_fft(struct dm_axis_fft *p, const void *in, void *out)
     p->out_slot = bo.slot;
   }
   void *s = dm::world_stream(comm->world);
-  for (auto &L : p->passes) dm::launch_fft_pass(L, s);
+  for (auto &L : p->passes)
+    if (!dm::launch_ok(p->in->comm, dm::launch_fft_pass(L, s), "FFT pass launch")) return DM_ERR_ARG;
   dm::world_finish(comm->world);
   return DM_SUCCESS;
 }

@@ -298,9 +298,9 @@ extern "C" int dm_batch_run(DmBatch b, void *stream) {
     L.primes = b->primes;
     L.grid = p.grid;
     L.lds = b->lds;
-    dm::launch_batch_pass(L, stream);
+    if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run: batched pass launch")) return 1;
   }
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  return 0;
 }
 
 extern "C" int dm_batch_graph_launch(DmBatch b, void *stream) {

@@ -218,14 +218,16 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
 }
 
 template <typename R, bool INV, int CAP, bool PRIMES>
-void launch_typed(const BatchLaunch &p, hipStream_t s) {
+hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
   auto k = batch_pass_kernel<R, INV, CAP, BNTH, PRIMES>;
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
     attr = true;
   }
   hipLaunchKernelGGL(k, dim3((unsigned)p.grid), dim3(BNTH), p.lds, s, p);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -256,26 +258,18 @@ long long batch_resident_groups(int dbl, int primes, size_t lds) {
   return (long long)cus * std::max(1, std::min(by_regs, by_lds));
 }
 
-void launch_batch_pass(const BatchLaunch &p, void *stream) {
-  if (p.ntiles <= 0 || p.grid <= 0) return;
+int launch_batch_pass(const BatchLaunch &p, void *stream) {
+  if (p.ntiles <= 0 || p.grid <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
   if (p.dbl) {
-    if (p.primes) {
-      if (p.inverse) launch_typed<double, true, BCAP_D, true>(p, s);
-      else launch_typed<double, false, BCAP_D, true>(p, s);
-    } else {
-      if (p.inverse) launch_typed<double, true, BCAP_D, false>(p, s);
-      else launch_typed<double, false, BCAP_D, false>(p, s);
-    }
+    if (p.primes) e = p.inverse ? launch_typed<double, true, BCAP_D, true>(p, s) : launch_typed<double, false, BCAP_D, true>(p, s);
+    else e = p.inverse ? launch_typed<double, true, BCAP_D, false>(p, s) : launch_typed<double, false, BCAP_D, false>(p, s);
   } else {
-    if (p.primes) {
-      if (p.inverse) launch_typed<float, true, BCAP_F, true>(p, s);
-      else launch_typed<float, false, BCAP_F, true>(p, s);
-    } else {
-      if (p.inverse) launch_typed<float, true, BCAP_F, false>(p, s);
-      else launch_typed<float, false, BCAP_F, false>(p, s);
-    }
+    if (p.primes) e = p.inverse ? launch_typed<float, true, BCAP_F, true>(p, s) : launch_typed<float, false, BCAP_F, true>(p, s);
+    else e = p.inverse ? launch_typed<float, true, BCAP_F, false>(p, s) : launch_typed<float, false, BCAP_F, false>(p, s);
   }
+  return (int)e;
 }
 
 }  // namespace dm

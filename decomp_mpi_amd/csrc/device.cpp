@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "internal.h"
+#include "kernels.h"
 
 namespace dm {
 
@@ -67,6 +68,12 @@ void world_finish(dm_world *w) {
   if (!w->async) stream_sync(world_stream(w));
 }
 
+bool launch_ok(dm_comm_t comm, int err, const char *what) {
+  if (err == 0) return true;
+  fail(comm, "%s failed: %s", what, hipGetErrorString((hipError_t)err));
+  return false;
+}
+
 void destroy_stream(void *stream) {
   hipStreamSynchronize((hipStream_t)stream);
   hipStreamDestroy((hipStream_t)stream);
@@ -86,9 +93,6 @@ extern "C" int dm_fill(void *dst, int value, size_t bytes_per_rank) {
   return DM_SUCCESS;
 }
 
-namespace dm {
-void launch_copy16(const void *src, void *dst, long long n16, void *stream);
-}
 
 // Plain device-to-device copy of `bytes` (a multiple of 16) on the world's
 // stream, timed with HIP events; returns milliseconds (negative on error).
@@ -99,14 +103,15 @@ extern "C" double dm_copy_bench(MPI_Comm comm, void *dst, const void *src, size_
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  dm::launch_copy16(src, dst, (long long)(bytes / 16), s);
+  bool ok = dm::launch_ok(comm, dm::launch_copy16(src, dst, (long long)(bytes / 16), s), "copy kernel launch");
   (void)hipEventRecord(e0, s);
-  for (int i = 0; i < reps; i++) dm::launch_copy16(src, dst, (long long)(bytes / 16), s);
+  for (int i = 0; i < reps && ok; i++)
+    ok = dm::launch_ok(comm, dm::launch_copy16(src, dst, (long long)(bytes / 16), s), "copy kernel launch");
   (void)hipEventRecord(e1, s);
   (void)hipEventSynchronize(e1);
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return ms / reps;
+  return ok ? ms / reps : -1.0;
 }

@@ -83,6 +83,22 @@ std::vector<double> roots(int n) {
   return w;
 }
 
+bool axis_supported(int n, int dbl, std::string *why) {
+  char msg[160];
+  std::vector<int> rad;
+  const int cap = fft_tile_capacity(dbl, 1);
+  if (n > cap) {
+    snprintf(msg, sizeof msg, "axis length %d exceeds the %d-point tile of the %s pass", n, cap,
+             dbl ? "complex-double" : "complex-float");
+  } else if (!choose_radices(n, &rad) || (int)rad.size() > 16) {
+    snprintf(msg, sizeof msg, "axis length %d has a prime factor above 13", n);
+  } else {
+    return true;
+  }
+  if (why) *why = msg;
+  return false;
+}
+
 size_t TableBuilder::put(const void *src, size_t bytes) {
   size_t at = host.size();
   host.resize(at + ((bytes + 255) & ~(size_t)255), 0);

@@ -1,6 +1,7 @@
 // Building FFT pass programs from layouts (shared by the split plan, the
 // per-axis FFT and the batched transforms).
 #pragma once
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -18,6 +19,9 @@ struct Side {
 Side make_side(const HostLayout &h, i64 slot);
 
 bool choose_radices(int n, std::vector<int> *out);
+// Whether the pass kernel transforms lines of n points in this precision
+// (n within one tile, radices 2..16, 3, 5, 7, 11, 13); why: the reason if not.
+bool axis_supported(int n, int dbl, std::string *why);
 // exp(-2 pi i k / n), k < n, interleaved (re, im), accurate to double rounding
 std::vector<double> roots(int n);
 

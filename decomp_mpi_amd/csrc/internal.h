@@ -64,6 +64,16 @@ namespace dm {
 void fail(dm_comm_t comm, const char *fmt, ...);  // raise_error with formatting
 bool error_pending();
 int error_mode();
+// Error mode of the calling thread for the lifetime of the object.
+struct ScopedErrorMode {
+  explicit ScopedErrorMode(int mode);
+  ~ScopedErrorMode();
+  ScopedErrorMode(const ScopedErrorMode &) = delete;
+  ScopedErrorMode &operator=(const ScopedErrorMode &) = delete;
+
+ private:
+  int saved_;
+};
 
 // ---- device runtime (device.cpp)
 bool device_available();
@@ -76,6 +86,9 @@ void *world_stream(dm_world *w);
 void world_finish(dm_world *w);   // synchronize unless async
 void stream_sync(void *stream);
 void destroy_stream(void *stream);
+// err: a launcher's return value (a hipError_t).  Raises "<what> failed:
+// <HIP error>" and returns false when it is not 0.
+bool launch_ok(dm_comm_t comm, int err, const char *what);
 
 // ---- buffer registry (world.cpp)
 struct Buffer {

@@ -31,7 +31,8 @@ struct ReshapeLaunch {
   int elem_bytes;
   int nontemporal;            // 1: non-temporal hint on the element accesses
 };
-void launch_reshape(const ReshapeLaunch &p, void *stream);
+// Launchers return 0 or the HIP error code of the launch (hipError_t).
+int launch_reshape(const ReshapeLaunch &p, void *stream);
 
 // Stage s of a line of N points: nr = N / radix butterflies per line, ns =
 // product of the earlier radices (Stockham span), magics for both.
@@ -90,7 +91,7 @@ struct FftLaunch {
   int stream_in, stream_out;  // 1: non-temporal hint on the data loads / stores
 };
 // nblocks == 0: the launch covers every tile of the pass.
-void launch_fft_pass(const FftLaunch &p, void *stream);
+int launch_fft_pass(const FftLaunch &p, void *stream);
 // Largest points per tile the FFT pass kernel holds for an element size:
 // row walks a half tile, column walks the full one, and complex-double
 // column walks at a large line stride ("wide") a double one (16 lines of
@@ -126,7 +127,8 @@ struct BatchLaunch {
   long long grid;              // persistent workgroups (tile loop inside)
   size_t lds;
 };
-void launch_batch_pass(const BatchLaunch &p, void *stream);
+int launch_batch_pass(const BatchLaunch &p, void *stream);
+int launch_copy16(const void *src, void *dst, long long n16, void *stream);
 // Host tables of a batch (batch.cpp): tiles[pass] (last axis first),
 // schedules, their stage-ordered roots as complex doubles, LDS bytes.
 struct BatchHost {

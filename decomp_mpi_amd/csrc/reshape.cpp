@@ -385,8 +385,7 @@ bool enqueue_reshape(dm_comm_t comm, ReshapePlan plan, const void *src, void *ds
   static int nont = -1;
   if (nont < 0) nont = getenv("DM_NO_NONTEMPORAL") != nullptr;
   L.nontemporal = nont ? 0 : 1;
-  launch_reshape(L, stream);
-  return true;
+  return launch_ok(comm, launch_reshape(L, stream), "reshape kernel launch");
 }
 }  // namespace dm
 

@@ -244,34 +244,35 @@ __global__ __launch_bounds__(NTHREADS) void reshape_run1d(ReshapeLaunch p) {
 }
 
 template <int BYTES, bool US, bool UD, bool NT>
-void launch_hinted(const ReshapeLaunch &p, hipStream_t s) {
+hipError_t launch_hinted(const ReshapeLaunch &p, hipStream_t s) {
   if (p.a != p.b)
     hipLaunchKernelGGL((reshape_tile2d<BYTES, US, UD, TILE_SLICES, NT>), dim3((unsigned)p.nblocks),
                        dim3(NTHREADS), 0, s, p);
   else
     hipLaunchKernelGGL((reshape_run1d<BYTES, US, UD, NT>), dim3((unsigned)p.nblocks), dim3(NTHREADS), 0, s, p);
+  return hipGetLastError();
 }
 
 template <int BYTES, bool US, bool UD>
-void launch_typed(const ReshapeLaunch &p, hipStream_t s) {
-  if (p.nblocks <= 0) return;
-  if (p.nontemporal) launch_hinted<BYTES, US, UD, true>(p, s);
-  else launch_hinted<BYTES, US, UD, false>(p, s);
+hipError_t launch_typed(const ReshapeLaunch &p, hipStream_t s) {
+  if (p.nblocks <= 0) return hipSuccess;
+  if (p.nontemporal) return launch_hinted<BYTES, US, UD, true>(p, s);
+  return launch_hinted<BYTES, US, UD, false>(p, s);
 }
 
 template <int BYTES>
-void launch_bytes(const ReshapeLaunch &p, hipStream_t s) {
+hipError_t launch_bytes(const ReshapeLaunch &p, hipStream_t s) {
   bool us = p.src.uniform, ud = p.dst.uniform;
-  if (us && ud) launch_typed<BYTES, true, true>(p, s);
-  else if (us) launch_typed<BYTES, true, false>(p, s);
-  else if (ud) launch_typed<BYTES, false, true>(p, s);
-  else launch_typed<BYTES, false, false>(p, s);
+  if (us && ud) return launch_typed<BYTES, true, true>(p, s);
+  if (us) return launch_typed<BYTES, true, false>(p, s);
+  if (ud) return launch_typed<BYTES, false, true>(p, s);
+  return launch_typed<BYTES, false, false>(p, s);
 }
 
 }  // namespace
 
 // Tile counts and grid size are filled here so the host only chooses axes.
-void launch_reshape(const ReshapeLaunch &p0, void *stream) {
+int launch_reshape(const ReshapeLaunch &p0, void *stream) {
   ReshapeLaunch p = p0;
   hipStream_t s = (hipStream_t)stream;
   long long batch = 1;
@@ -287,12 +288,12 @@ void launch_reshape(const ReshapeLaunch &p0, void *stream) {
     p.nblocks = (long long)p.tiles_a * batch;
   }
   switch (p.elem_bytes) {
-    case 1: launch_bytes<1>(p, s); break;
-    case 2: launch_bytes<2>(p, s); break;
-    case 4: launch_bytes<4>(p, s); break;
-    case 8: launch_bytes<8>(p, s); break;
-    case 16: launch_bytes<16>(p, s); break;
-    default: break;  // other sizes are split into words by the caller
+    case 1: return (int)launch_bytes<1>(p, s);
+    case 2: return (int)launch_bytes<2>(p, s);
+    case 4: return (int)launch_bytes<4>(p, s);
+    case 8: return (int)launch_bytes<8>(p, s);
+    case 16: return (int)launch_bytes<16>(p, s);
+    default: return (int)hipErrorInvalidValue;  // other sizes are split into words by the caller
   }
 }
 
@@ -310,10 +311,11 @@ __global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ s
 }
 }  // namespace
 
-void launch_copy16(const void *src, void *dst, long long n16, void *stream) {
+int launch_copy16(const void *src, void *dst, long long n16, void *stream) {
   long long blocks = (n16 + 1023) / 1024;
-  if (blocks <= 0) return;
+  if (blocks <= 0) return 0;
   hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      (const uint4 *)src, (uint4 *)dst, n16);
+  return (int)hipGetLastError();
 }
 }  // namespace dm

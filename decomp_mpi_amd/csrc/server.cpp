@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdarg>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -67,6 +69,19 @@ size_t size_class(size_t b) {
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// A refused request is the caller's to handle (submit returns -1): the
+// reason goes to dm_last_error without the abort of error mode 0.
+void reject(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void reject(const char *fmt, ...) {
+  char msg[256];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(msg, sizeof msg, fmt, ap);
+  va_end(ap);
+  dm::ScopedErrorMode record(1);
+  raise_error(nullptr, "%s", msg);
+}
 
 }  // namespace
 
@@ -135,7 +150,7 @@ extern "C" long long dm_server_submit(DmServer s, int ndim, const int *shape, in
                                       const void *in, void *out) {
   if (s == nullptr || shape == nullptr || in == nullptr || out == nullptr || ndim < 1 || ndim > 3 ||
       (direction != -1 && direction != 1)) {
-    raise_error(nullptr, "dm_server_submit: bad arguments");
+    reject("dm_server_submit: bad arguments");
     return -1;
   }
   Req r;
@@ -148,7 +163,7 @@ extern "C" long long dm_server_submit(DmServer s, int ndim, const int *shape, in
     r.shape[d] = shape[d];
     if (shape[d] < 1 || shape[d] > dm::batch_tile_capacity(r.dbl) || !dm::choose_radices(shape[d], &rad) ||
         rad.size() > 16) {
-      raise_error(nullptr, "dm_server_submit: axis length %d is not supported", shape[d]);
+      reject("dm_server_submit: axis length %d is not supported", shape[d]);
       return -1;
     }
     n *= (size_t)shape[d];
@@ -158,7 +173,7 @@ extern "C" long long dm_server_submit(DmServer s, int ndim, const int *shape, in
   r.bytes = n * (r.dbl ? 16 : 8);
   r.cls = size_class(r.bytes);
   if (r.cls > s->budget) {
-    raise_error(nullptr, "dm_server_submit: request of %zu bytes exceeds the budget", r.bytes);
+    reject("dm_server_submit: request of %zu bytes exceeds the budget", r.bytes);
     return -1;
   }
   r.t_sub = s->now();

@@ -23,7 +23,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <numeric>
+#include <string>
 
 #include "internal.h"
 #include "kernels.h"
@@ -400,6 +402,10 @@ static bool compile_plan(SplitPlan plan) {
 
 // ---------------------------------------------------------------- planner
 
+// Why the calling thread's last plan was rejected for an unsupported axis
+// (added to makeSplitPlan's "Cannot generate plan" message).
+static thread_local std::string t_rejection;
+
 static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, Decomp start, int ndim, int A[],
                             int B[], int L[], int grid_a[], int grid_b[], void *dat, int length, void *workspace) {
   if ((flags & FORCE_USE_FIRST) && !use_first) return nullptr;
@@ -413,6 +419,17 @@ static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, D
       raise_error(comm, "Invalid factor");
       return nullptr;
     }
+  // The device passes transform whole axes: an axis they cannot do rejects
+  // the plan here, as the reference rejects plans it cannot build
+  // (split_fft.c:252-254,304-308): a logged reason and NULL.
+  for (int i = 0; i < ndim; i++) {
+    std::string why;
+    if (L[i] > 1 && !dm::axis_supported(L[i], dbl, &why)) {
+      logger(comm, "buildParallelPlan: axis %d: %s\n", i, why.c_str());
+      t_rejection = "axis " + std::to_string(i) + ": " + why;
+      return nullptr;
+    }
+  }
   SplitPlan plan = (SplitPlan)calloc(1, sizeof(struct split_fft_plan));
   plan->comm = comm;
   plan->src = (fftw_complex *)dat;
@@ -574,8 +591,12 @@ static SplitPlan split_plan(int dbl, int slab, Decomp start, void *dat, int leng
     }
     B[i] = BB;
   }
+  t_rejection.clear();
   SplitPlan r = propose(dbl, NO_FLAGS, start, B.data(), dat, length, workspace);
-  if (r == nullptr) raise_error(start->comm, "Cannot generate plan");
+  if (r == nullptr) {
+    if (t_rejection.empty()) raise_error(start->comm, "Cannot generate plan");
+    else raise_error(start->comm, "Cannot generate plan (%s)", t_rejection.c_str());
+  }
   return r;
 }
 
@@ -613,19 +634,21 @@ static double time_plan(SplitPlan p, int reps) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
+  bool ok = true;
   auto run = [&]() {
-    for (auto &L : p->ext->fwd.passes) dm::launch_fft_pass(L, s);
+    for (auto &L : p->ext->fwd.passes)
+      if (ok) ok = dm::launch_ok(p->comm, dm::launch_fft_pass(L, s), "FFT pass launch");
   };
   run();
   (void)hipEventRecord(e0, (hipStream_t)s);
-  for (int i = 0; i < reps; i++) run();
+  for (int i = 0; i < reps && ok; i++) run();
   (void)hipEventRecord(e1, (hipStream_t)s);
   (void)hipEventSynchronize(e1);
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return ms / reps;
+  return ok ? ms / reps : 1e30;
 }
 
 static SplitPlan autotune(int dbl, Decomp start, void *dat, int length, void *workspace) {
@@ -642,8 +665,9 @@ static SplitPlan autotune(int dbl, Decomp start, void *dat, int length, void *wo
   };
   std::vector<Cand> cands;
   std::vector<int> B(ndim, 0);
-  int em = dm::error_mode();
-  dm_set_error_mode(1);  // candidates that cannot be planned are skipped
+  // candidates that cannot be planned are skipped: errors are recorded, not
+  // raised, on this thread while the candidates are built
+  std::unique_ptr<dm::ScopedErrorMode> quiet(new dm::ScopedErrorMode(1));
   std::function<void(int)> rec = [&](int d) {
     if (d == ndim) {
       bool all_one = true;
@@ -674,7 +698,7 @@ static SplitPlan autotune(int dbl, Decomp start, void *dat, int length, void *wo
     }
     rec(0);
   }
-  dm_set_error_mode(em);
+  quiet.reset();
   if (cands.empty()) return nullptr;
   std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.cost < b.cost; });
   size_t keep = std::min<size_t>(cands.size(), 4);
@@ -707,11 +731,17 @@ static void run_program(SplitPlan plan, dm_fft_program &prog, int t_first, int t
     for (auto &e : ev) (void)hipEventCreate(&e);
     (void)hipEventRecord(ev[0], (hipStream_t)s);
   }
-  for (size_t i = 0; i < n; i++) {
-    dm::launch_fft_pass(prog.passes[i], s);
+  bool ok = true;
+  for (size_t i = 0; i < n && ok; i++) {
+    ok = dm::launch_ok(plan->comm, dm::launch_fft_pass(prog.passes[i], s), "FFT pass launch");
     if (timed) (void)hipEventRecord(ev[i + 1], (hipStream_t)s);
   }
   if (!timed) return;
+  if (!ok) {
+    (void)hipStreamSynchronize((hipStream_t)s);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return;
+  }
   (void)hipEventSynchronize(ev[n]);
   double total = 0;
   for (size_t i = 0; i < n; i++) {

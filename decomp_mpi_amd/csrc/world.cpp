@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -118,20 +119,31 @@ extern "C" int dm_type_size(dm_datatype_t type, int *size) {
 
 // ---------------------------------------------------------------- errors
 
-static int g_error_mode = 0;
-static std::string g_last_error;
-static bool g_error_pending = false;
+// The error mode is a process-wide default that a thread may override for a
+// scope (the autotuner does, around candidates that may not plan); the last
+// error and the pending flag belong to the thread that raised them, so the
+// rank threads of a dm_spmd_run session never write one shared string.
+static std::atomic<int> g_error_mode{0};
+static thread_local int t_error_mode = -1;  // -1: follow the process default
+static thread_local std::string t_last_error;
+static thread_local bool t_error_pending = false;
 
-extern "C" void dm_set_error_mode(int mode) { g_error_mode = mode; }
-extern "C" const char *dm_last_error(void) { return g_last_error.c_str(); }
+extern "C" void dm_set_error_mode(int mode) {
+  g_error_mode.store(mode);
+  t_error_mode = -1;
+}
+extern "C" const char *dm_last_error(void) { return t_last_error.c_str(); }
 extern "C" void dm_clear_error(void) {
-  g_last_error.clear();
-  g_error_pending = false;
+  t_last_error.clear();
+  t_error_pending = false;
 }
 
 namespace dm {
-bool error_pending() { return g_error_pending; }
-int error_mode() { return g_error_mode; }
+bool error_pending() { return t_error_pending; }
+int error_mode() { return t_error_mode >= 0 ? t_error_mode : g_error_mode.load(); }
+
+ScopedErrorMode::ScopedErrorMode(int mode) : saved_(t_error_mode) { t_error_mode = mode; }
+ScopedErrorMode::~ScopedErrorMode() { t_error_mode = saved_; }
 
 static void vraise(dm_comm_t comm, const char *fmt, va_list ap) {
   char msg[1024];
@@ -139,9 +151,9 @@ static void vraise(dm_comm_t comm, const char *fmt, va_list ap) {
   char line[1200];
   // Same text as the reference's raise_error (reshape.c:18).
   snprintf(line, sizeof line, "%d: Error in reshape:%s\n", comm ? comm->rank : 0, msg);
-  if (g_error_mode == 1) {
-    g_last_error = line;
-    g_error_pending = true;
+  if (error_mode() == 1) {
+    t_last_error = line;
+    t_error_pending = true;
     return;
   }
   fputs(line, stderr);

@@ -1,0 +1,211 @@
+"""Host-side pieces of the split-FFT plan and the index lists, no GPU needed:
+makeIndexList (reshape.c:104-136) against a brute-force run list, the
+plan's twiddle tables (split_fft.c:348-420) and decomp_scale
+(dimscale.c:7-67) against the oracle for every view rank, visitData
+(split_fft.c:862-889) against the decomposition's own placement, and the
+plan-time rejection of axes the device passes cannot transform."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import decomp_mpi_amd as dm
+import oracle
+import refmodel as rm
+from helpers import prod_dim
+from test_host import HostBuffers
+
+
+@pytest.fixture(scope="module")
+def C():
+    return dm.lib()
+
+
+def index_runs(ptr):
+    ix = ctypes.cast(ptr, ctypes.POINTER(dm.IndexS)).contents
+    return [(ix.offsets[i], ix.counts[i]) for i in range(ix.count)]
+
+
+def brute_runs(mine, my_pos, remote, procs):
+    """For ascending g held at my_pos in `mine`: append mine's local offset
+    to the list of remote's owner of g, merging an offset that extends the
+    previous run by one."""
+    lists = [[] for _ in range(procs)]
+    for g in range(rm.coord_size(mine)):
+        pos, off = rm.rank_off(mine, g)
+        if pos != my_pos:
+            continue
+        rr, _ = rm.rank_off(remote, g)
+        runs = lists[rr]
+        if runs and runs[-1][0] + runs[-1][1] == off:
+            runs[-1] = (runs[-1][0], runs[-1][1] + 1)
+        else:
+            runs.append((off, 1))
+    return lists
+
+
+INDEX_CASES = [
+    (("block", 12, 3), ("block", 12, 4)),
+    (("block", 16, 2), ("interleave", ("block", 4, 2), 4)),
+    (("rotate", 5, ("block", 12, 3)), ("block", 12, 2)),
+    (("interleave", ("block", 4, 2), 3), ("frev", 3, 4, ("block", 12, 3))),
+    (("local", 10), ("block", 10, 3)),
+]
+
+
+@pytest.mark.parametrize("mine,remote", INDEX_CASES, ids=str)
+def test_index_list_matches_brute_force(C, mine, remote):
+    """makeIndexList: per remote rank, the runs of my local offsets in
+    ascending global order, NULL-terminated."""
+    procs_m = rm.proc_size(mine)
+    comm = C.dm_world_create(procs_m)
+    for r in range(procs_m):
+        view = C.dm_comm_view(comm, r)
+        ax = C.makeAxisDecomp(view, prod_dim(C, mine), 1, 1)
+        rd = prod_dim(C, remote)
+        lst = C.makeIndexList(ax, rd)
+        assert lst
+        arr = ctypes.cast(lst, ctypes.POINTER(ctypes.c_void_p))
+        want = brute_runs(mine, r, remote, rm.proc_size(remote))
+        got = [index_runs(arr[i]) for i in range(len(want))]
+        assert arr[len(want)] is None
+        assert got == want, (r, got, want)
+        C.freeIndexList(lst)
+        C.freeDimDecomp(rd)
+        C.freeAxisDecomp(ax)
+    C.dm_world_free(comm)
+
+
+SPLITS = [((16, 8, 12), [4, 2, 3], 4), ((16, 16, 16), [2, 4, 8], 8), ((12, 10, 8), [3, 5, 2], 6)]
+
+
+def plan_for_rank(C, comm, r, shape, B, P):
+    """A plan built on rank r's view of a block start decomposition."""
+    grid = oracle.processor_grid(P, list(shape))
+    view = C.dm_comm_view(comm, r)
+    start = C.makePackedDecomp(view, 3, dm.ptrs([prod_dim(C, ("block", n, g)) for n, g in zip(shape, grid)]))
+    per = max(oracle.Decomp(P, [(oracle.Block(n, g), 1) for n, g in zip(shape, grid)]).extents)
+    length = 4 * int(np.prod(shape)) + 64
+    dat, work = HostBuffers(C, comm, P, per), HostBuffers(C, comm, P, length)
+    plan = C.makeParallelPlan(start, dm.ints(B), dat.ptr, length, work.ptr)
+    assert plan
+    return plan, start, dat, work
+
+
+@pytest.mark.parametrize("shape,B,P", SPLITS)
+def test_plan_twiddles_and_decomp_scale_match_oracle(C, shape, B, P):
+    """plan->twiddle[i] is the oracle's table for the view rank (exact
+    quarter turns, the rest within 2 ulp), and decomp_scale with those
+    tables scales a rank's A-layout block as the oracle does, sign -1
+    (forward) and +1 (conjugate)."""
+    A = [n // b for n, b in zip(shape, B)]
+    ga, gb = oracle.processor_grid(P, A), oracle.processor_grid(P, B)
+    Ad, _ = oracle.split_decomps(P, A, B, ga, gb)
+    comm = C.dm_world_create(P)
+    for r in range(P):
+        plan, start, dat, work = plan_for_rank(C, comm, r, shape, B, P)
+        sp = dm.split_plan(plan)
+        assert [sp.A_factors[i] for i in range(3)] == A
+        tw = ctypes.cast(sp.twiddle, ctypes.POINTER(ctypes.c_void_p))
+        tabs = []
+        for i in range(3):
+            want = oracle.twiddle_table(P, Ad, r, A, list(shape), i)
+            got = np.ctypeslib.as_array(ctypes.cast(tw[i], ctypes.POINTER(ctypes.c_double)),
+                                        (2 * len(want),)).view(np.complex128).copy()
+            exact = (np.abs(want.real) == 1) | (np.abs(want.imag) == 1) | (want == 0)
+            assert np.array_equal(got[exact], want[exact])
+            assert np.array_equal(np.signbit(got[exact].real), np.signbit(want[exact].real))
+            assert np.allclose(got, want, rtol=0, atol=4.5e-16)
+            tabs.append(got)
+        ext = int(Ad.extents[r])
+        rng = np.random.default_rng(r)
+        x = rng.uniform(-1, 1, ext) + 1j * rng.uniform(-1, 1, ext)
+        for sign in (-1, 1):
+            mine, ref = x.copy(), x.copy()
+            C.decomp_scale(sp.A, sp.twiddle, mine.ctypes.data, sign)
+            oracle.decomp_scale(P, Ad, r, A, list(shape), ref.view(np.float64), sign)
+            assert np.allclose(mine, ref, rtol=0, atol=1e-15)
+            assert not np.array_equal(mine, x)
+        C.freeSplitPlan(plan)
+        dat.free()
+        work.free()
+        C.freeDecomp(start)
+    C.dm_world_free(comm)
+
+
+VISITOR = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p)
+
+
+def test_visit_data_reports_every_slot(C):
+    """visitData calls the visitor once per slot of the rank's local
+    bounding box; a slot that holds a coordinate gets that coordinate, the
+    others -1 on the axes where they hold none."""
+    P, shape, grid = 6, (10, 7, 9), (2, 3, 1)
+    comm = C.dm_world_create(P)
+    od = oracle.Decomp(P, [(oracle.Block(n, g), rs) for n, g, rs in zip(shape, grid, (3, 1, 1))])
+    for r in range(P):
+        view = C.dm_comm_view(comm, r)
+        d = C.makePackedDecomp(view, 3, dm.ptrs([prod_dim(C, ("block", n, g)) for n, g in zip(shape, grid)]))
+        ext = C.extent(d)
+        buf = np.zeros(ext, np.complex128)
+        base = buf.ctypes.data
+        seen = {}
+
+        def visit(nd, gpos, ptr):
+            seen[(ptr - base) // 16] = tuple(gpos[i] for i in range(nd))
+
+        C.visitData(d, VISITOR(visit), base)
+        assert len(seen) == ext
+        owned = {}
+        for g in np.ndindex(*shape):
+            rank, off = od.owner(g)
+            if rank == r:
+                owned[off] = g
+        full = {o: g for o, g in seen.items() if min(g) >= 0}
+        assert full == owned
+        C.freeDecomp(d)
+    C.dm_world_free(comm)
+
+
+@pytest.mark.parametrize("L,dbl,ok", [(272, 1, False), (17 * 8, 0, False), (8192, 1, False), (8192, 0, True),
+                                      (4096, 1, True), (16 * 27 * 5, 1, True)])
+def test_split_plan_rejects_unsupported_axis_at_plan_time(C, L, dbl, ok):
+    """An axis the device passes cannot transform (a prime factor above 13,
+    or longer than one tile) makes buildParallelPlan return NULL, as the
+    reference's other plan rejections do (split_fft.c:252-308), and
+    makeSplitPlan raise 'Cannot generate plan' with the reason; a supported
+    one plans."""
+    comm = C.dm_world_create(2)
+    start = C.makePackedDecomp(comm, 3, dm.ptrs([prod_dim(C, ("local", L)), prod_dim(C, ("block", 2, 2)),
+                                                 prod_dim(C, ("local", 1))]))
+    per = C.extent(start)
+    length = 4 * 2 * L + 64
+    dat, work = HostBuffers(C, comm, 2, per), HostBuffers(C, comm, 2, length)
+    b = max(q for q in range(1, L + 1) if L % q == 0 and q * q <= L)
+    A, B, Ls = [b, 1, 1], [L // b, 2, 1], [L, 2, 1]
+    ga, gb = oracle.processor_grid(2, A), oracle.processor_grid(2, B)
+    build = C.buildParallelPlan if dbl else C.buildParallelPlanf
+    make = C.makeSplitPlan if dbl else C.makeSplitPlanf
+    C.dm_set_error_mode(1)
+    try:
+        C.dm_clear_error()
+        p = build(0, 0, 0, start, 3, dm.ints(A), dm.ints(B), dm.ints(Ls), dm.ints(ga), dm.ints(gb),
+                  dat.ptr, length, work.ptr)
+        assert bool(p) == ok
+        assert C.dm_last_error() == b""   # a rejection, not an error
+        if p:
+            C.freeSplitPlan(p)
+        p = make(0, start, dat.ptr, length, work.ptr)
+        assert bool(p) == ok
+        if ok:
+            C.freeSplitPlan(p)
+        else:
+            msg = C.dm_last_error()
+            assert b"Cannot generate plan" in msg and b"axis 0" in msg, msg
+    finally:
+        C.dm_clear_error()
+        C.dm_set_error_mode(0)
+    dat.free()
+    work.free()
+    C.freeDecomp(start)
+    C.dm_world_free(comm)
