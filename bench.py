@@ -52,6 +52,63 @@ def make_input(seed=7):
     return x, modes
 
 
+def assemble(host_x):
+    """The global 512^3 array from the per-rank pencil blocks."""
+    ny, nz = L // GRID[1], L // GRID[2]
+    glob = np.empty((L, L, L), np.complex128)
+    for r in range(P):
+        ry, rz = r // GRID[2], r % GRID[2]
+        glob[:, ry * ny:(ry + 1) * ny, rz * nz:(rz + 1) * nz] = host_x[r].reshape(L, ny, nz)
+    return glob
+
+
+def host_threads():
+    """Host threads for the CPU legs: the GPU's share of the box (16) at most."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def fp64_reference(glob, reps=2):
+    """scipy.fft (pocketfft) in FP64 on the host threads: the independent
+    oracle spectrum of the global array, and the CPU stand-in for the
+    reference's MPI+FFTW path (FFTW is not installed), timed as forward +
+    inverse per step.  Returns (spectrum, seconds per step, threads)."""
+    import scipy.fft
+    w = host_threads()
+    X = None
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        X = scipy.fft.fftn(glob, workers=w)
+        back = scipy.fft.ifftn(X, workers=w)
+        del back
+    dt = (time.perf_counter() - t0) / reps
+    return X, dt, w
+
+
+def spectrum_checks(spec, X, modes, tol):
+    """Relative L2 against the FP64 oracle spectrum and the planted peaks
+    (|X[k]| = 10 N at each planted wavenumber, within 1%)."""
+    N = L ** 3
+    err = float(np.linalg.norm(spec - X) / np.linalg.norm(X))
+    peaks = [abs(abs(spec[tuple(k)]) / (10.0 * N) - 1) for k in modes]
+    ok = err < tol and max(peaks) < 0.01
+    return err, float(max(peaks)), ok
+
+
 def setup(C, dm, comm, dbl, host_x, factors=None):
     dims = [C.makeLocalDecomp(L), C.makeBlockDecomp(L, GRID[1]), C.makeBlockDecomp(L, GRID[2])]
     start = C.makePackedDecomp(comm, 3, dm.ptrs(dims))
@@ -286,37 +343,78 @@ def measured_traffic(bytes_per_launch):
             "source": "profiles/fft_pass_512cube_c128_traffic.json"}
 
 
+TIMER_NAMES = ["RESHAPE_1", "FFT_1", "TWIDDLE", "RESHAPE_2", "FFT_2", "ALL"]
+
+
 def time_steps(C, dm, comm, plan, steps, warmup):
+    """Warm-up, then exactly `steps` forward+inverse steps between two
+    device synchronisations.  Per step: the HIP-event time of its passes
+    (plan->times[ALL] accumulates the per-pass event intervals, recorded on
+    the stream the passes run on).  Returns (wall seconds, per-step event
+    seconds, the plan's Timers per step)."""
     sp = dm.split_plan(plan)
     for _ in range(warmup):
         C.runSplitPlan(plan, dm.FFTW_FORWARD)
         C.runSplitPlan(plan, dm.FFTW_BACKWARD)
     for i in range(dm.NUM_TIMERS):
         sp.times[i] = 0.0
+    per_step = []
     C.dm_synchronize(comm)
     t0 = time.perf_counter()
     for _ in range(steps):
+        before = sp.times[dm.ALL]
         C.runSplitPlan(plan, dm.FFTW_FORWARD)
         C.runSplitPlan(plan, dm.FFTW_BACKWARD)
+        per_step.append(sp.times[dm.ALL] - before)
     C.dm_synchronize(comm)
     wall = time.perf_counter() - t0
-    kernel_s = sp.times[dm.ALL]      # sum of per-pass HIP event times
-    return wall, kernel_s
+    timers = {name: round(sp.times[i] / steps * 1e3, 4) for i, name in enumerate(TIMER_NAMES)}
+    return wall, np.array(per_step), timers
 
 
-def cpu_baseline(n=512):
-    """The oracle (C restatement of the reference path, one core): one
-    forward transform of the same 512^3 pipeline on the same 4x8 grid with
-    the reference's default split (A = 16, B = 32), about 20 s of CPU work;
-    GB/s at the same algorithmic 6 * N * 16 bytes per transform."""
+def step_stats(per_step):
+    ms = per_step * 1e3
+    return {"median": round(float(np.median(ms)), 4), "min": round(float(ms.min()), 4),
+            "mean": round(float(ms.mean()), 4), "n": int(ms.size)}
+
+
+def cpu_reshape_baseline():
+    """The oracle's reshape (the reference's message pairing restated in C,
+    one host core) on the 512^3 int64 chain: GB/s per hop at 2 * N * 8
+    bytes."""
+    import oracle
+    shifts = [int(v) for v in np.random.default_rng(11).integers(0, L, 3)]
+    block = oracle.Decomp(P, [(oracle.Block(L, 4), 8), (oracle.Block(L, 8), 1), (oracle.Local(L), 1)],
+                          strides=[(L // 8) * L, L, 1])
+    rot = oracle.Decomp(P, [(oracle.Rotate(shifts[0], oracle.Block(L, 4)), 8),
+                            (oracle.Rotate(shifts[1], oracle.Block(L, 8)), 1),
+                            (oracle.Rotate(shifts[2], oracle.Local(L)), 1)],
+                        strides=[L // 8, 1, (L // 4) * (L // 8)])
+    inter = oracle.Decomp(P, [(oracle.Interleave(oracle.Block(4, 4), L // 4), 8), (oracle.Block(L, 8), 1),
+                              (oracle.Local(L), 1)])
+    ext = [int(max(d.extents)) for d in (block, rot, inter)]
+    bufs = [np.arange(P * ext[0], dtype=np.int64).reshape(P, ext[0])]
+    bufs += [np.zeros((P, e), np.int64) for e in ext[1:]]
+    secs = []
+    for k, (a, b) in enumerate(((block, rot), (rot, inter))):
+        t0 = time.perf_counter()
+        oracle.reshape(a, b, bufs[k].view(np.uint8), bufs[k + 1].view(np.uint8), 8)
+        secs.append(time.perf_counter() - t0)
+    alg = 2 * L ** 3 * 8
+    return {"GB_per_s": [round(alg / t / 1e9, 4) for t in secs], "s_per_hop": [round(t, 3) for t in secs],
+            "cores": 1, "kind": "port",
+            "sample": "oracle reshape (the reference's per-message pairing restated in C), both hops of the "
+                      "512^3 int64 chain, one host core"}
+
+
+def cpu_split_port_baseline(n=512):
+    """The oracle's split FFT (the reference's two stages restated in C,
+    radix-2 stage DFTs, one core), one forward transform of the 512^3 array
+    on the same 4x8 grid with the reference's default split."""
     import oracle
     start = oracle.Decomp(P, [(oracle.Local(n), 32), (oracle.Block(n, 4), 8), (oracle.Block(n, 8), 1)])
-    a = 1
-    for q in range(1, n):
-        if n % q == 0 and q * q <= n:
-            a = q
-    A = [a] * 3
-    B = [n // a] * 3
+    a = max(q for q in range(1, n) if n % q == 0 and q * q <= n)
+    A, B = [a] * 3, [n // a] * 3
     ga, gb = oracle.processor_grid(P, A), oracle.processor_grid(P, B)
     ext = oracle.split_extent(P, A, B, ga, gb)
     per = int(max(start.extents))
@@ -326,8 +424,7 @@ def cpu_baseline(n=512):
     t0 = time.perf_counter()
     oracle.split_fft(start, A, B, ga, gb, x, w, ext, -1)
     dt = time.perf_counter() - t0
-    alg = 6 * n ** 3 * 16
-    return {"value": round(alg / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+    return {"GB_per_s": round(6 * n ** 3 * 16 / dt / 1e9, 4), "cores": 1, "kind": "port",
             "sample": f"oracle split FFT, one forward transform, {n}^3 complex double, 4x8 virtual grid, "
                       f"A={A}, {dt:.2f} s on one host core"}
 
@@ -335,9 +432,11 @@ def cpu_baseline(n=512):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the FP64 reference spectrum and the error checks (profiling runs)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-reshape", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -350,40 +449,64 @@ def main():
                                    "the processor grid (multi-GPU is a design, not a run)"}))
         sys.exit(2)
     import decomp_mpi_amd as dm
+    from decomp_mpi_amd.layout import flat_index
     C = dm.lib()
     if not C.dm_device_available():
         raise SystemExit("no HIP device")
     print(f"bench: generating {L}^3 input", file=sys.stderr, flush=True)
     host_x, modes = make_input()
-    comm = C.dm_world_create(P)
     N = L ** 3
     alg_step = 2 * 6 * N * 16
+    X = None
+    cpu_fft = None
+    if not args.no_check:
+        print("bench: FP64 reference spectrum (scipy.fft on the host)", file=sys.stderr, flush=True)
+        glob = assemble(host_x)
+        X, cpu_step_s, threads = fp64_reference(glob)
+        del glob
+        cpu_fft = {"value": round(alg_step / cpu_step_s / 1e9, 4), "unit": "GB/s", "cores": threads,
+                   "kind": "port",
+                   "sample": f"scipy.fft (pocketfft) FP64 fftn + ifftn of the same 512^3 array on {threads} host "
+                             f"threads, 2 steps, {cpu_step_s:.2f} s per step; a library stand-in for the "
+                             "reference's MPI+FFTW path (FFTW is not installed)",
+                   "backend": "scipy.fft / pocketfft (FP64)", "cpu_model": cpu_model(),
+                   "ms_per_step": round(cpu_step_s * 1e3, 1)}
+    comm = C.dm_world_create(P)
+    factors = [int(v) for v in args.factors.split(",")] if args.factors else None
+    checks_ok = True
+
+    def verify(plan, dat, work, per, dbl):
+        """Forward against the FP64 spectrum and the planted peaks, then
+        the round trip over every rank; leaves the input in dat."""
+        ct, es, tol = (np.complex128, 16, 1e-12) if dbl else (np.complex64, 8, 1e-6)
+        sp_ = dm.split_plan(plan)
+        C.runSplitPlan(plan, dm.FFTW_FORWARD)
+        spec = work.get_all(sp_.extent * es, ct).ravel()[flat_index(C.getTransformed(plan), sp_.extent)]
+        err, peak, ok = spectrum_checks(spec.reshape(L, L, L).astype(np.complex128), X, modes, tol)
+        del spec
+        C.runSplitPlan(plan, dm.FFTW_BACKWARD)
+        back = dat.get_all(per * es, ct).astype(np.complex128) / N
+        rt = float(np.linalg.norm(back - host_x) / np.linalg.norm(host_x))
+        del back
+        dat.put_all(host_x if dbl else host_x.astype(np.complex64))
+        return {"rel_l2_vs_fp64_dft": err, "max_planted_peak_rel_dev": peak, "round_trip_rel_l2_all_ranks": rt,
+                "tolerance": tol, "ok": bool(ok and rt < tol)}
 
     print("bench: planning FP64", file=sys.stderr, flush=True)
-    factors = [int(v) for v in args.factors.split(",")] if args.factors else None
     start, dat, work, plan, per = setup(C, dm, comm, True, host_x, factors)
     sp = dm.split_plan(plan)
     A64 = [sp.A_factors[i] for i in range(3)]
-    # correctness spot checks on the timed path: planted peaks and round trip
-    C.runSplitPlan(plan, dm.FFTW_FORWARD)
-    C.runSplitPlan(plan, dm.FFTW_BACKWARD)
-    back = dat.get(0, per * 16, np.complex128) / N
-    rt64 = float(np.linalg.norm(back - host_x[0]) / np.linalg.norm(host_x[0]))
-    dat.put_all(host_x)
+    chk64 = None
+    if X is not None:
+        chk64 = verify(plan, dat, work, per, True)
+        checks_ok = checks_ok and chk64["ok"]
     print("bench: timing FP64", file=sys.stderr, flush=True)
-    wall, kern = time_steps(C, dm, comm, plan, args.steps, args.warmup)
+    wall, per_step, timers64 = time_steps(C, dm, comm, plan, args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3
     passes = 6 * args.steps
-    pass_ms = kern / passes * 1e3
+    pass_ms = float(per_step.sum()) / passes * 1e3
     pass_bytes = 2 * N * 16
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
-    # forward result for the FP32 error figure, in frequency order (the two
-    # precisions' autotuned plans may place the output differently)
-    from decomp_mpi_amd.layout import flat_index
-    dat.put_all(host_x)
-    C.runSplitPlan(plan, dm.FFTW_FORWARD)
-    ext = sp.extent
-    ref_fwd = work.get_all(ext * 16, np.complex128).ravel()[flat_index(C.getTransformed(plan), ext)]
     C.freeSplitPlan(plan)
     dat.free()
     work.free()
@@ -394,24 +517,21 @@ def main():
         start32, dat32, work32, plan32, _ = setup(C, dm, comm, False, host_x, factors)
         sp32 = dm.split_plan(plan32)
         A32 = [sp32.A_factors[i] for i in range(3)]
-        wall32, kern32 = time_steps(C, dm, comm, plan32, args.steps, args.warmup)
-        dat32.put_all(host_x.astype(np.complex64))
-        C.runSplitPlan(plan32, dm.FFTW_FORWARD)
-        ext32 = sp32.extent
-        got32 = work32.get_all(ext32 * 8, np.complex64).ravel()[flat_index(C.getTransformed(plan32), ext32)]
-        err32 = float(np.linalg.norm(got32.astype(np.complex128) - ref_fwd) / np.linalg.norm(ref_fwd))
-        del got32
-        C.runSplitPlan(plan32, dm.FFTW_BACKWARD)
-        back32 = dat32.get(0, per * 8, np.complex64).astype(np.complex128) / N
-        rt32 = float(np.linalg.norm(back32 - host_x[0]) / np.linalg.norm(host_x[0]))
+        chk32 = None
+        if X is not None:
+            chk32 = verify(plan32, dat32, work32, per, False)
+            checks_ok = checks_ok and chk32["ok"]
+        wall32, per_step32, timers32 = time_steps(C, dm, comm, plan32, args.steps, args.warmup)
         fp32 = {"ms_per_step": round(wall32 / args.steps * 1e3, 4),
                 "GB_per_s": round(alg_step / (wall32 / args.steps) / 1e9, 2),
                 "speedup_vs_fp64": round(wall / wall32, 3),
-                "rel_l2_vs_fp64": err32, "round_trip_rel_l2": rt32, "A_factors": A32,
-                "pass_ms": round(kern32 / passes * 1e3, 4)}
+                "step_ms_events": step_stats(per_step32), "timers_ms_per_step": timers32,
+                "checks": chk32, "A_factors": A32,
+                "pass_ms": round(float(per_step32.sum()) / passes * 1e3, 4)}
         C.freeSplitPlan(plan32)
         dat32.free()
         work32.free()
+    X = None
 
     # plain 2 GiB device copy: the achievable-bandwidth reference
     cbuf = dm.DistBuffer(comm, per * 16 * 2)
@@ -434,7 +554,12 @@ def main():
         print("bench: serving loop", file=sys.stderr, flush=True)
         serving = serving_bench(C, dm)
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = cpu_fft
+    if cpu is not None and not args.no_cpu:
+        print("bench: CPU ports (oracle reshape chain, oracle split FFT sample)", file=sys.stderr, flush=True)
+        cpu = dict(cpu)
+        cpu["reshape_chain_port"] = cpu_reshape_baseline()
+        cpu["split_fft_port"] = cpu_split_port_baseline()
     value = alg_step / (wall / args.steps) / 1e9
     out = {
         "metric": "512^3 c2c 3D FFT time & HBM GB/s, FP32 vs FP64 form; batched req/s + p99",
@@ -452,7 +577,8 @@ def main():
         "config": {"workload": "split FFT forward+inverse, 512^3 complex double, pencil start "
                                "{Local(512), Block(512,4) rs8, Block(512,8) rs1} on 32 virtual ranks",
                    "A_factors": A64, "gflops": round(2 * 5 * N * np.log2(N) / (wall / args.steps) / 1e9, 1),
-                   "round_trip_rel_l2": rt64, "fp32_form": fp32, "reshape_chain": chain,
+                   "step_ms_events": step_stats(per_step), "timers_ms_per_step": timers64,
+                   "checks": chk64, "fp32_form": fp32, "reshape_chain": chain,
                    "batched": batched, "serving": serving},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -463,7 +589,10 @@ def main():
                      "copy_2GiB_GBps": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if not checks_ok:
+        print("bench: a result check failed (see config.checks)", file=sys.stderr, flush=True)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

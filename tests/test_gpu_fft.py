@@ -29,9 +29,10 @@ def block_start(shape, grid):
     return [("block", n, g) for n, g in zip(shape, grid)]
 
 
-def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_descs=None):
+def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_descs=None, autotune=False):
     """Forward then backward on the device; forward checked against the
-    oracle on every slot of the output layout, backward as N * x."""
+    oracle on every slot of the output layout and (through getTransformed)
+    against numpy's FP64 DFT of the global array, backward as N * x."""
     comm = C.dm_world_create(P)
     descs = start_descs or block_start(shape, grid)
     ps, os_ = make_pair(C, comm, P, descs)
@@ -41,7 +42,9 @@ def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_desc
     length = 2 * max(per, N // P + 1) * 4 + 64
     dat = dm.DistBuffer(comm, per * es)
     work = dm.DistBuffer(comm, length * es)
-    if slab is not None:
+    if autotune:
+        plan = (C.autotuneParallelPlan if dbl else C.autotuneParallelPlanf)(ps, dat.ptr, length, work.ptr)
+    elif slab is not None:
         plan = (C.makeSplitPlan if dbl else C.makeSplitPlanf)(slab, ps, dat.ptr, length, work.ptr)
     else:
         plan = (C.makeParallelPlan if dbl else C.makeParallelPlanf)(ps, dm.ints(B), dat.ptr, length, work.ptr)
@@ -67,6 +70,17 @@ def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_desc
     out_d = oracle.split_decomps(P, A, Bf, ga, gb)[1 if any(a > 1 for a in A) else 0]
     mask = owned_mask(out_d, ext)
     fwd_err = rel_l2(got[mask], ow[:, :ext][mask])
+    # independently of the oracle: numpy's DFT of the global array, with
+    # frequency k of axis d at output position k / B_d + A_d * (k % B_d)
+    # (A = 1 on every axis: the single-stage output in the A layout)
+    sr, so = placement(os_)
+    glob = x[sr, so]
+    grids = np.meshgrid(*[np.arange(n) // b + a * (np.arange(n) % b) for n, a, b in zip(shape, A, Bf)],
+                        indexing="ij")
+    orank, ooff = placement(out_d)
+    pos = tuple(grids)
+    spec = got[orank[pos], ooff[pos]]
+    fwd_err = max(fwd_err, rel_l2(spec, np.fft.fftn(glob)))
     # backward: unnormalised inverse restores N * x on the start layout
     C.runSplitPlan(plan, dm.FFTW_BACKWARD)
     back = dat.get_all(per * es, dt).astype(np.complex128)
@@ -79,13 +93,31 @@ def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_desc
     return fwd_err, rt_err, A
 
 
+def placement(d):
+    """(rank, local offset) arrays over every global coordinate of the
+    oracle decomposition d (per-rank strides), from per-axis tables."""
+    n = d.ndim
+    rank = np.zeros([1] * n, np.int64)
+    parts = []
+    for i, (spec, rs) in enumerate(d.axes):
+        tab = np.array([oracle.rank_offset(spec, g) for g in range(d.L[i])], np.int64)
+        shp = [1] * n
+        shp[i] = d.L[i]
+        rank = rank + (tab[:, 0] * rs).reshape(shp)
+        parts.append(tab[:, 1].reshape(shp))
+    rank = np.broadcast_to(rank, d.L)
+    assert np.all(rank < d.proc_used)
+    off = np.zeros(d.L, np.int64)
+    for i in range(n):
+        off = off + d.strides[rank, i] * parts[i]
+    return rank, off
+
+
 def owned_mask(d, width):
     """Slots of each rank that hold a coordinate of decomposition d."""
     mask = np.zeros((d.P, width), bool)
-    for coord in np.ndindex(*d.L):
-        r, off = d.owner(coord)
-        if r is not None:
-            mask[r, off] = True
+    r, off = placement(d)
+    mask[r, off] = True
     return mask
 
 
@@ -98,6 +130,10 @@ def owned_mask(d, width):
     (6, (12, 20, 14), (1, 2, 3), [3, 4, 7]),      # radices 3, 5, 7
     (4, (16, 16), (2, 2), None),                  # 2D
     (4, (16, 16, 16), (1, 2, 2), [16, 16, 16]),   # single stage (A = 1)
+    (4, (16, 16, 16), (1, 2, 2), [1, 1, 8]),      # B_d = 1 beside B_d > 1
+    (8, (16, 16, 16), (2, 2, 2), [1, 4, 1]),
+    (8, (16, 16, 16), (1, 2, 4), [1, 1, 16]),     # the 512^3 bench family, scaled down
+    (32, (64, 64, 64), (1, 4, 8), [1, 1, 32]),
 ])
 def test_split_fft_fp64_vs_oracle(C, P, shape, grid, B):
     if B is None:
@@ -116,6 +152,8 @@ def test_split_fft_slab(C):
 @pytest.mark.parametrize("P,shape,grid,B", [
     (4, (16, 16, 16), (1, 2, 2), None),
     (8, (32, 16, 32), (2, 2, 2), [4, 4, 8]),
+    (8, (16, 16, 16), (2, 2, 2), [1, 4, 1]),
+    (8, (16, 16, 16), (1, 2, 4), [1, 1, 16]),
 ])
 def test_split_fft_fp32_vs_fp64_oracle(C, P, shape, grid, B):
     if B is None:
@@ -124,6 +162,20 @@ def test_split_fft_fp32_vs_fp64_oracle(C, P, shape, grid, B):
         fwd, rt, _ = run_split(C, P, shape, grid, B=B, dbl=False)
     assert fwd < TOL32, f"FP32 forward rel L2 {fwd:.3e}"
     assert rt < TOL32
+
+
+@pytest.mark.parametrize("dbl", [True, False])
+@pytest.mark.parametrize("P,shape,grid", [(4, (16, 16, 16), (1, 2, 2)), (8, (32, 16, 32), (2, 2, 2)),
+                                          (32, (64, 64, 64), (1, 4, 8))])
+def test_autotuned_plan_vs_oracle_and_dft(C, P, shape, grid, dbl):
+    """Whatever autotuneParallelPlan[f] returns (split_fft.c:78-247 picks by
+    timing; this library by a traffic model plus timing), its forward
+    matches the oracle run with the plan's own factors and numpy's DFT, and
+    its backward restores N * x."""
+    fwd, rt, A = run_split(C, P, shape, grid, dbl=dbl, autotune=True)
+    tol = TOL64 if dbl else TOL32
+    assert fwd < tol, f"forward rel L2 {fwd:.3e} (A={A})"
+    assert rt < tol, f"round trip {rt:.3e} (A={A})"
 
 
 def test_pencil_axis_ffts_64(C):

@@ -6,10 +6,11 @@
   element holds its own global linear index, so the check is exact and
   size-independent: each destination slot holds the index its descriptor
   prescribes, and every slot no coordinate maps to keeps its fill value.
-* The 512^3 complex-double split FFT (pencil start on 4x8, default split
-  A=16, B=32), forward against numpy/scipy's FP64 DFT (relative L2 < 1e-12),
-  the planted modes, and the unnormalised round trip; the FP32 form within
-  1e-6 of the same FP64 spectrum.
+* The 512^3 complex-double split FFT (pencil start on 4x8) with the
+  default split (A=16, B=32), the mixed split B=(1,1,256) and the
+  autotuned plan, forward against scipy's FP64 DFT (relative L2 < 1e-12),
+  the planted modes, and the unnormalised round trip on every rank; the
+  FP32 form within 1e-6 of the same FP64 spectrum.
 """
 import numpy as np
 import pytest
@@ -112,8 +113,16 @@ def c3_data():
     return x, dft3(x), modes
 
 
+PLANS = [("default_B32", [32, 32, 32]), ("B_1_1_256", [1, 1, 256]), ("autotuned", None)]
+
+
 @pytest.mark.parametrize("dbl", [1, 0])
-def test_split_fft_512_vs_fp64_dft(C, c3_data, dbl):
+@pytest.mark.parametrize("name,factors", PLANS, ids=[p[0] for p in PLANS])
+def test_split_fft_512_vs_fp64_dft(C, c3_data, dbl, name, factors):
+    """The reference's default split (A = 16, B = 32 on every axis), the
+    mixed family B = (1, 1, 256) and whatever autotuneParallelPlan[f]
+    returns (the plan bench.py times), each against the FP64 DFT, the
+    planted modes and the round trip on every rank."""
     x, X, modes = c3_data
     comm = C.dm_world_create(P)
     start = C.makePackedDecomp(comm, 3, dm.ptrs([C.makeLocalDecomp(L), C.makeBlockDecomp(L, 4),
@@ -129,11 +138,23 @@ def test_split_fft_512_vs_fp64_dft(C, c3_data, dbl):
     host[sidx.ravel()] = x.ravel()
     dat.put_all(host.reshape(P, per))
     del host
-    make = C.makeParallelPlan if dbl else C.makeParallelPlanf
-    plan = make(start, dm.ints([32, 32, 32]), dat.ptr, length, work.ptr)
+    if factors is None:
+        tune = C.autotuneParallelPlan if dbl else C.autotuneParallelPlanf
+        plan = tune(start, dat.ptr, length, work.ptr)
+        # the autotuner timed candidates on dat: put the input back
+        host = np.zeros(P * per, ct)
+        host[sidx.ravel()] = x.ravel()
+        dat.put_all(host.reshape(P, per))
+        del host
+    else:
+        make = C.makeParallelPlan if dbl else C.makeParallelPlanf
+        plan = make(start, dm.ints(factors), dat.ptr, length, work.ptr)
     assert plan
     sp = dm.split_plan(plan)
-    assert [sp.A_factors[i] for i in range(3)] == [16, 16, 16]
+    A = [sp.A_factors[i] for i in range(3)]
+    if factors is not None:
+        assert A == [L // b for b in factors]
+    print(f"plan {name}: A = {A}", flush=True)
     ext = sp.extent
     C.runSplitPlan(plan, dm.FFTW_FORWARD)
     got = work.get_all(ext * es, ct).ravel()
@@ -141,7 +162,7 @@ def test_split_fft_512_vs_fp64_dft(C, c3_data, dbl):
     del got
     tol = 1e-12 if dbl else 1e-6
     err = rel_l2(spec, X)
-    assert err < tol, f"forward relative L2 {err:.3e}"
+    assert err < tol, f"forward relative L2 {err:.3e} (A = {A})"
     for k in modes:
         assert abs(abs(spec[tuple(k)]) / (10.0 * N) - 1) < 0.01
     del spec

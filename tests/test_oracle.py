@@ -278,3 +278,17 @@ def test_single_stage_split_is_plain_dft():
     spec, back, _ = run_oracle_split(2, x, [1, 1, 1], [1, 2, 1])
     assert rel(spec, np.fft.fftn(x)) < 1e-13
     assert rel(back / x.size, x) < 1e-13
+
+
+@pytest.mark.parametrize("P,shape,grid,A", [(4, (16, 16, 16), (1, 2, 2), [16, 16, 2]),
+                                            (8, (16, 16, 16), (2, 2, 2), [16, 4, 16]),
+                                            (8, (16, 16, 16), (1, 2, 4), [16, 16, 1]),
+                                            (8, (8, 12, 16), (2, 2, 2), [1, 3, 16])])
+def test_split_fft_mixed_unit_factors_matches_numpy(P, shape, grid, A):
+    """Plans with B_d = 1 on some axes beside B_d > 1 on others (the family
+    B = (1, 1, 256) that the 512^3 autotuner picks): the oracle still equals
+    numpy's DFT, placed by k -> k/B + A (k%B), and round-trips."""
+    x = np.random.default_rng(5).uniform(-1, 1, shape) + 1j * np.random.default_rng(6).uniform(-1, 1, shape)
+    spec, back, _ = run_oracle_split(P, x, A, list(grid))
+    assert rel(spec, np.fft.fftn(x)) < 1e-13
+    assert rel(back / x.size, x) < 1e-13
