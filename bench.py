@@ -274,26 +274,46 @@ def serving_bench(C, dm, count=1000, streams=4, max_batch=8, budget=64 << 30):
             C.dm_server_poll(s)
         C.dm_server_drain(s)
         sub, fin = ctypes.c_double(), ctypes.c_double()
-        times = []
+        times, parts = [], []
+        ms = (ctypes.c_double * 6)()
         for rid in ids:
             C.dm_server_times(s, rid, ctypes.byref(sub), None, ctypes.byref(fin))
             times.append((sub.value, fin.value))
+            C.dm_server_breakdown(s, rid, ms)
+            parts.append(list(ms))
         nb, mx = ctypes.c_longlong(), ctypes.c_longlong()
         peak = ctypes.c_size_t()
         C.dm_server_stats(s, ctypes.byref(nb), ctypes.byref(mx), ctypes.byref(peak), None)
+        cached, builds = ctypes.c_longlong(), ctypes.c_longlong()
+        C.dm_server_plan_stats(s, ctypes.byref(cached), ctypes.byref(builds), None)
         C.dm_server_destroy(s)
-        return times, nb.value, mx.value, peak.value
+        return times, nb.value, mx.value, peak.value, np.array(parts), builds.value
 
     # closed loop: everything at time 0
     n0 = min(256, count)
-    times, _, _, _ = run(probe[:n0], [0.0] * n0)
+    times = run(probe[:n0], [0.0] * n0)[0]
     capacity = n0 / (max(f for _, f in times) - min(sub for sub, _ in times))
     lam = 0.6 * capacity
     reqs = workloads.serving_requests(count, lam, seed=5)
     arrivals = [a for a, _, _, _ in reqs]
-    times, batches, max_seen, peak = run(reqs, arrivals)
+    times, batches, max_seen, peak, parts, builds = run(reqs, arrivals)
     first = min(sub for sub, _ in times)
     lat = np.array([fin - (first + arr) for (sub, fin), arr in zip(times, arrivals)]) * 1e3
+    # where the time went: mean breakdown of the requests at or above p99,
+    # and of those around the median; "arrival_lag" is the submit time after
+    # the scheduled arrival (the driver loop's own delay)
+    lag = np.array([(sub - first - arr) * 1e3 for (sub, _), arr in zip(times, arrivals)])
+    names = ["queue", "host_enqueue", "copy_in", "kernels", "copy_out", "completion_lag"]
+
+    def mean_parts(sel):
+        out = {n: round(float(parts[sel, i].mean()), 3) for i, n in enumerate(names)}
+        out["arrival_lag"] = round(float(lag[sel].mean()), 3)
+        out["bytes_mean"] = int(np.mean([np.prod(reqs[j][1]) * (16 if reqs[j][2] else 8)
+                                         for j in np.nonzero(sel)[0]]))
+        return out
+    p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+    breakdown = {"at_or_above_p99": mean_parts(lat >= p99),
+                 "around_p50": mean_parts((lat >= np.percentile(lat, 40)) & (lat <= np.percentile(lat, 60)))}
     span = max(f for _, f in times) - first
     # accuracy per precision class on a separate mixed set
     acc = {0: 0.0, 1: 0.0}
@@ -322,10 +342,11 @@ def serving_bench(C, dm, count=1000, streams=4, max_batch=8, budget=64 << 30):
                         f"batches of up to {max_batch}",
             "closed_loop_capacity_req_per_s": round(capacity, 1), "lambda_req_per_s": round(lam, 1),
             "throughput_req_per_s": round(count / span, 1),
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                           "p99": round(float(np.percentile(lat, 99)), 3),
+            "latency_ms": {"p50": round(float(p50), 3), "p99": round(float(p99), 3),
                            "mean": round(float(lat.mean()), 3)},
+            "latency_breakdown_ms": breakdown,
             "batches": batches, "largest_batch": max_seen, "peak_device_bytes": peak,
+            "plans_built": builds,
             "max_rel_l2_vs_fp64": {"c64": acc[0], "c128": acc[1]}}
 
 

@@ -137,6 +137,9 @@ _SIGS = {
     "dm_batch_bytes": (D, [P]), "dm_batch_free": (None, [P]),
     # request loop (dm_serve.h)
     "dm_server_create": (P, [SZ, I, I]), "dm_server_submit": (ctypes.c_longlong, [P, I, IP, I, I, P, P]),
+    "dm_server_breakdown": (I, [P, ctypes.c_longlong, ctypes.POINTER(D)]),
+    "dm_server_plan_stats": (I, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(SZ)]),
     "dm_server_poll": (I, [P]), "dm_server_wait": (I, [P, ctypes.c_longlong]), "dm_server_drain": (I, [P]),
     "dm_server_times": (I, [P, ctypes.c_longlong, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)]),
     "dm_server_stats": (I, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),

@@ -121,17 +121,35 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   constexpr int V = CAP / NTH;
-  const cx<R> *twall = reinterpret_cast<const cx<R> *>(p.twiddle);
   for (long long t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
-    const BatchTile d = p.tiles[t];
-    const BatchSched &S = p.sched[d.sched];
+    // the tile, its schedule, roots and transform: from the launch's tables,
+    // or from the segment (one transform's own tables) that holds tile t
+    BatchTile d;
+    const BatchSched *sched;
+    const cx<R> *twall;
+    void *xdata;
+    if (p.nseg > 0) {
+      int sg = 0;
+      while (sg + 1 < p.nseg && t >= p.seg[sg].end) sg++;
+      const long long first = sg > 0 ? p.seg[sg - 1].end : 0;
+      d = p.seg[sg].tiles[t - first];
+      sched = p.seg[sg].sched;
+      twall = reinterpret_cast<const cx<R> *>(p.seg[sg].twiddle);
+      xdata = p.seg[sg].data;
+    } else {
+      d = p.tiles[t];
+      sched = p.sched;
+      twall = reinterpret_cast<const cx<R> *>(p.twiddle);
+      xdata = p.data[d.xform];
+    }
+    const BatchSched &S = sched[d.sched];
     const int N = S.N, T = S.T, nl = d.nl;
     const int tot = T * N;
     const int ls = lds_line(N);
     const long long lstride = d.row ? (long long)N : 1LL;
     // the transform's pointer comes from memory, so its address space is
     // stated: through a generic pointer every access would be a flat one
-    gptr<R> data = (gptr<R>)(p.data[d.xform]) + d.base;
+    gptr<R> data = (gptr<R>)(xdata) + d.base;
     const cx<R> *tw = twall + S.tw_off;
     // Fused first / last stages: a power-of-two radix dividing the points per
     // thread (whole butterflies per thread); the last one needs two stages.

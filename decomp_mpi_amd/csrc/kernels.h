@@ -116,6 +116,17 @@ struct BatchSched {
   unsigned long long m_n, m_t;
   StageInfo st[16];
 };
+// A segment: the tiles of one transform whose tables live apart from the
+// others' (a device-resident per-shape plan of the request loop).  Its
+// tiles have xform 0 and address that transform's data directly.
+struct BatchSeg {
+  const BatchTile *tiles;
+  const BatchSched *sched;
+  const void *twiddle;
+  void *data;
+  long long end;               // launch tiles [previous end, end) are this segment's
+};
+#define DM_BATCH_MAX_SEG 16
 struct BatchLaunch {
   const BatchTile *tiles;
   long long ntiles;
@@ -126,6 +137,8 @@ struct BatchLaunch {
   int primes;                  // some schedule has a radix 5, 7, 11 or 13
   long long grid;              // persistent workgroups (tile loop inside)
   size_t lds;
+  int nseg;                    // > 0: tiles come from seg[] (tiles/sched/data/twiddle unused)
+  BatchSeg seg[DM_BATCH_MAX_SEG];
 };
 int launch_batch_pass(const BatchLaunch &p, void *stream);
 int launch_copy16(const void *src, void *dst, long long n16, void *stream);

@@ -2,13 +2,22 @@
 //
 // State of a request: queued -> admitted (device buffer from the pool,
 // within the HBM budget) -> launched (part of a batch on a stream) ->
-// finished (the stream's event for that batch has completed).  Each poll
-// retires finished batches, admits queued requests in arrival order while
-// they fit, and gives every idle stream a batch: the oldest admitted
+// finished (the stream's last event for that batch has completed).  Each
+// poll retires finished batches, admits queued requests in arrival order
+// while they fit, and gives every idle stream a batch: the oldest admitted
 // request plus the following admitted requests of the same precision,
-// direction and rank, up to max_batch.  A batch is: table upload (from a
-// pinned staging area into the stream's device arena), input copies, one
-// batched pass launch per axis (dm_batch kernels), output copies, event.
+// direction and rank, up to max_batch.
+//
+// Plans are device-resident: the first request of a (shape, precision,
+// direction) builds that shape's tables once -- radix schedules, their
+// twiddles and the tile list of every axis pass -- and uploads them; later
+// requests of the shape reuse them (least recently used plans go when the
+// cache is full).  A batch is then: input copies, one batched pass launch
+// per axis whose kernel argument lists each request's tables and data as a
+// segment (nothing is rebuilt or uploaded per batch), output copies.  Four
+// timing events per batch give each request's copy-in, kernel and copy-out
+// time beside its queue and host time.
+//
 // A request's transform depends only on its own shape (tile heights and
 // radix schedules are functions of the line length), so its result is the
 // same alone and in any mix.
@@ -22,6 +31,7 @@
 #include <deque>
 #include <map>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "dm_batch.h"
@@ -36,6 +46,19 @@ using Clock = std::chrono::steady_clock;
 
 enum State { QUEUED, ADMITTED, LAUNCHED, FINISHED };
 
+// Tables of one shape on the device: schedules | roots | tiles of each pass.
+struct Plan {
+  void *dev = nullptr;
+  size_t bytes = 0;
+  size_t o_sched = 0, o_tw = 0, o_tiles[3] = {0, 0, 0};
+  long long ntiles[3] = {0, 0, 0};
+  size_t lds = 0;
+  int primes = 0;
+  int refs = 0;             // requests admitted with this plan and not yet finished
+  long long last_use = 0;
+};
+typedef std::tuple<int, int, int, int, int, int> PlanKey;  // ndim, shape[3], dbl, inverse
+
 struct Req {
   int ndim = 0;
   int shape[3] = {1, 1, 1};
@@ -44,19 +67,17 @@ struct Req {
   void *out = nullptr;
   size_t bytes = 0, cls = 0;
   void *dev = nullptr;
+  Plan *plan = nullptr;
   State state = QUEUED;
   double t_sub = 0, t_launch = 0, t_done = 0;
+  double host_ms = 0, h2d_ms = 0, kern_ms = 0, d2h_ms = 0;
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // start, inputs in, kernels done, outputs out
   bool busy = false;
   std::vector<long long> reqs;
-  void *arena = nullptr;   // device tables of the batch in flight
-  size_t arena_cap = 0;
-  char *stage = nullptr;   // pinned host image of those tables
-  size_t stage_cap = 0;
 };
 
 // Buffer size classes: powers of two up to 2 MiB, then multiples of 2 MiB.
@@ -83,6 +104,9 @@ void reject(const char *fmt, ...) {
   raise_error(nullptr, "%s", msg);
 }
 
+constexpr size_t kPlanCacheEntries = 256;
+constexpr size_t kPlanCacheBytes = (size_t)256 << 20;
+
 }  // namespace
 
 struct dm_server {
@@ -92,6 +116,9 @@ struct dm_server {
   std::vector<Req> reqs;
   std::deque<long long> queue, ready;
   std::map<size_t, std::vector<void *>> pool;  // free device buffers by class
+  std::map<PlanKey, Plan *> plans;
+  size_t plan_bytes = 0;
+  long long plan_builds = 0, plan_clock = 0;
   Clock::time_point t0 = Clock::now();
   long long batches = 0, max_seen = 0;
   long long unfinished = 0;
@@ -121,13 +148,89 @@ static void *pool_get(DmServer s, size_t cls) {
 
 static void pool_put(DmServer s, void *p, size_t cls) { s->pool[cls].push_back(p); }
 
+static void plan_free(Plan *p) {
+  dm::device_free(p->dev);
+  delete p;
+}
+
+// Drop least recently used plans that no request holds until the cache is
+// within its entry and byte limits.
+static void plan_trim(DmServer s) {
+  while (s->plans.size() > kPlanCacheEntries || s->plan_bytes > kPlanCacheBytes) {
+    auto victim = s->plans.end();
+    for (auto it = s->plans.begin(); it != s->plans.end(); ++it)
+      if (it->second->refs == 0 && (victim == s->plans.end() || it->second->last_use < victim->second->last_use))
+        victim = it;
+    if (victim == s->plans.end()) return;  // every cached plan is in use
+    s->plan_bytes -= victim->second->bytes;
+    plan_free(victim->second);
+    s->plans.erase(victim);
+  }
+}
+
+// The device tables of a request's shape, with a reference taken for the
+// caller: from the cache, or built and uploaded once.  Null (after an
+// error message) if the device refuses.
+static Plan *plan_for(DmServer s, const Req &r) {
+  PlanKey key(r.ndim, r.shape[0], r.shape[1], r.shape[2], r.dbl, r.inv);
+  auto it = s->plans.find(key);
+  if (it != s->plans.end()) {
+    it->second->last_use = ++s->plan_clock;
+    it->second->refs++;
+    return it->second;
+  }
+  dm::BatchHost h;
+  if (!dm::build_batch_host(1, r.ndim, r.shape, r.dbl, DM_BATCH_TILE_FULL, DM_BATCH_RADIX_LARGE_FIRST, &h)) {
+    raise_error(nullptr, "dm_server: cannot plan a transform of this shape");
+    return nullptr;
+  }
+  Plan *p = new Plan;
+  p->o_sched = 0;
+  p->o_tw = align256(h.sched.size() * sizeof(dm::BatchSched));
+  const size_t tw_bytes = (h.tw.size() / 2) * (r.dbl ? 16 : 8);
+  size_t at = p->o_tw + align256(tw_bytes);
+  for (int d = 0; d < r.ndim; d++) {
+    p->o_tiles[d] = at;
+    p->ntiles[d] = (long long)h.tiles[d].size();
+    at += align256(h.tiles[d].size() * sizeof(dm::BatchTile));
+  }
+  std::vector<char> img(at, 0);
+  memcpy(img.data() + p->o_sched, h.sched.data(), h.sched.size() * sizeof(dm::BatchSched));
+  if (r.dbl) {
+    memcpy(img.data() + p->o_tw, h.tw.data(), tw_bytes);
+  } else {
+    float *f = (float *)(img.data() + p->o_tw);
+    for (size_t i = 0; i < h.tw.size(); i++) f[i] = (float)h.tw[i];
+  }
+  for (int d = 0; d < r.ndim; d++)
+    memcpy(img.data() + p->o_tiles[d], h.tiles[d].data(), h.tiles[d].size() * sizeof(dm::BatchTile));
+  p->dev = dm::device_alloc(at);
+  if (p->dev == nullptr || hipMemcpy(p->dev, img.data(), at, hipMemcpyHostToDevice) != hipSuccess) {
+    dm::device_free(p->dev);
+    delete p;
+    raise_error(nullptr, "dm_server: device allocation or upload of a plan (%zu bytes) failed", at);
+    return nullptr;
+  }
+  p->bytes = at;
+  p->lds = h.lds;
+  p->primes = h.primes;
+  p->last_use = ++s->plan_clock;
+  p->refs = 1;  // held by the caller, so the trim below keeps it
+  s->plans[key] = p;
+  s->plan_bytes += at;
+  s->plan_builds++;
+  plan_trim(s);
+  return p;
+}
+
 extern "C" DmServer dm_server_create(size_t budget, int streams, int max_batch) {
   if (!dm::device_available()) {
     raise_error(nullptr, "dm_server_create: no HIP device is available");
     return nullptr;
   }
-  if (budget == 0 || streams < 1 || streams > 16 || max_batch < 1) {
-    raise_error(nullptr, "dm_server_create: need budget > 0, 1 <= streams <= 16, max_batch >= 1");
+  if (budget == 0 || streams < 1 || streams > 16 || max_batch < 1 || max_batch > DM_BATCH_MAX_SEG) {
+    raise_error(nullptr, "dm_server_create: need budget > 0, 1 <= streams <= 16, 1 <= max_batch <= %d",
+                DM_BATCH_MAX_SEG);
     return nullptr;
   }
   DmServer s = new dm_server;
@@ -136,8 +239,9 @@ extern "C" DmServer dm_server_create(size_t budget, int streams, int max_batch) 
   s->slots.resize(streams);
   for (auto &sl : s->slots) {
     // default priority, as every stream of this library
-    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess;
+    for (auto &e : sl.ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    if (!ok) {
       dm_server_destroy(s);
       raise_error(nullptr, "dm_server_create: stream or event creation failed");
       return nullptr;
@@ -184,30 +288,49 @@ extern "C" long long dm_server_submit(DmServer s, int ndim, const int *shape, in
   return id;
 }
 
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.0f;
+}
+
 static void retire(DmServer s) {
   for (auto &sl : s->slots) {
-    if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
+    if (!sl.busy || hipEventQuery(sl.ev[3]) != hipSuccess) continue;
     double t = s->now();
+    const float h2d = elapsed(sl.ev[0], sl.ev[1]), kern = elapsed(sl.ev[1], sl.ev[2]),
+                d2h = elapsed(sl.ev[2], sl.ev[3]);
     for (long long id : sl.reqs) {
       Req &r = s->reqs[id];
       r.state = FINISHED;
       r.t_done = t;
+      r.h2d_ms = h2d;
+      r.kern_ms = kern;
+      r.d2h_ms = d2h;
       pool_put(s, r.dev, r.cls);
       r.dev = nullptr;
+      r.plan->refs--;
+      r.plan = nullptr;
       s->in_use -= r.cls;
       s->unfinished--;
     }
     sl.reqs.clear();
     sl.busy = false;
   }
+  plan_trim(s);
 }
 
 static void admit(DmServer s) {
   while (!s->queue.empty()) {
     Req &r = s->reqs[s->queue.front()];
     if (s->in_use + r.cls > s->budget) break;  // arrival order: no overtaking
+    Plan *p = plan_for(s, r);
+    if (p == nullptr) break;
     r.dev = pool_get(s, r.cls);
-    if (r.dev == nullptr) break;  // device full: wait for buffers to come back
+    if (r.dev == nullptr) {  // device full: wait for buffers to come back
+      p->refs--;
+      break;
+    }
+    r.plan = p;
     s->in_use += r.cls;
     s->peak = std::max(s->peak, s->in_use);
     r.state = ADMITTED;
@@ -216,103 +339,77 @@ static void admit(DmServer s) {
   }
 }
 
-static bool grow(Slot &sl, size_t bytes) {
-  if (bytes <= sl.arena_cap && bytes <= sl.stage_cap) return true;
-  size_t cap = std::max(bytes, std::max(sl.arena_cap, sl.stage_cap) * 2);
-  dm::device_free(sl.arena);
-  if (sl.stage) (void)hipHostFree(sl.stage);
-  sl.arena = dm::device_alloc(cap);
-  void *h = nullptr;
-  if (hipHostMalloc(&h, cap, hipHostMallocDefault) != hipSuccess) h = nullptr;
-  sl.stage = (char *)h;
-  sl.arena_cap = sl.stage_cap = (sl.arena && sl.stage) ? cap : 0;
-  return sl.arena_cap >= bytes;
-}
-
 static bool launch(DmServer s, Slot &sl) {
-  // the oldest admitted request and the admitted ones that batch with it
+  const double t_host = s->now();
+  // the oldest admitted request and the admitted ones that batch with it;
+  // they leave the ready list only once the batch is enqueued
   const Req &head = s->reqs[s->ready.front()];
   std::vector<long long> ids;
-  for (auto it = s->ready.begin(); it != s->ready.end() && (int)ids.size() < s->max_batch;) {
+  for (auto it = s->ready.begin(); it != s->ready.end() && (int)ids.size() < s->max_batch; ++it) {
     const Req &r = s->reqs[*it];
-    if (r.dbl == head.dbl && r.inv == head.inv && r.ndim == head.ndim) {
-      ids.push_back(*it);
-      it = s->ready.erase(it);
-    } else {
-      ++it;
-    }
+    if (r.dbl == head.dbl && r.inv == head.inv && r.ndim == head.ndim) ids.push_back(*it);
   }
   const int nd = head.ndim, dbl = head.dbl, inv = head.inv;
-  std::vector<int> shapes;
-  std::vector<void *> ptrs;
-  for (long long id : ids) {
-    const Req &r = s->reqs[id];
-    shapes.insert(shapes.end(), r.shape, r.shape + nd);
-    ptrs.push_back(r.dev);
-  }
-  dm::BatchHost h;
-  if (!dm::build_batch_host((int)ids.size(), nd, shapes.data(), dbl, DM_BATCH_TILE_FULL,
-                            DM_BATCH_RADIX_LARGE_FIRST, &h))
-    return false;
-  // table image: pointers | schedules | roots | tiles of each pass
-  size_t o_ptr = 0, o_sched = align256(ptrs.size() * sizeof(void *));
-  size_t o_tw = o_sched + align256(h.sched.size() * sizeof(dm::BatchSched));
-  size_t tw_bytes = (h.tw.size() / 2) * (dbl ? 16 : 8);
-  std::vector<size_t> o_tiles(nd);
-  size_t at = o_tw + align256(tw_bytes);
-  for (int p = 0; p < nd; p++) {
-    o_tiles[p] = at;
-    at += align256(h.tiles[p].size() * sizeof(dm::BatchTile));
-  }
-  if (!grow(sl, at)) return false;
-  memcpy(sl.stage + o_ptr, ptrs.data(), ptrs.size() * sizeof(void *));
-  memcpy(sl.stage + o_sched, h.sched.data(), h.sched.size() * sizeof(dm::BatchSched));
-  if (dbl) {
-    memcpy(sl.stage + o_tw, h.tw.data(), tw_bytes);
-  } else {
-    float *f = (float *)(sl.stage + o_tw);
-    for (size_t i = 0; i < h.tw.size(); i++) f[i] = (float)h.tw[i];
-  }
-  for (int p = 0; p < nd; p++)
-    memcpy(sl.stage + o_tiles[p], h.tiles[p].data(), h.tiles[p].size() * sizeof(dm::BatchTile));
-  char *arena = (char *)sl.arena;
   hipStream_t st = sl.stream;
-  bool ok = hipMemcpyAsync(arena, sl.stage, at, hipMemcpyHostToDevice, st) == hipSuccess;
+  bool ok = hipEventRecord(sl.ev[0], st) == hipSuccess;
   for (long long id : ids) {
     const Req &r = s->reqs[id];
     ok = ok && hipMemcpyAsync(r.dev, r.in, r.bytes, hipMemcpyHostToDevice, st) == hipSuccess;
   }
-  long long resident = dm::batch_resident_groups(dbl, h.primes, h.lds);
-  for (int p = 0; p < nd; p++) {
+  ok = ok && hipEventRecord(sl.ev[1], st) == hipSuccess;
+  size_t lds = 0;
+  int primes = 0;
+  for (long long id : ids) {
+    lds = std::max(lds, s->reqs[id].plan->lds);
+    primes = primes || s->reqs[id].plan->primes;
+  }
+  const long long resident = dm::batch_resident_groups(dbl, primes, lds);
+  for (int d = 0; d < nd && ok; d++) {
     dm::BatchLaunch L{};
-    L.tiles = (const dm::BatchTile *)(arena + o_tiles[p]);
-    L.ntiles = (long long)h.tiles[p].size();
-    L.sched = (const dm::BatchSched *)(arena + o_sched);
-    L.data = (void *const *)(arena + o_ptr);
-    L.twiddle = arena + o_tw;
     L.dbl = dbl;
     L.inverse = inv;
-    L.primes = h.primes;
-    L.grid = std::min<long long>(L.ntiles, resident);
-    L.lds = h.lds;
-    dm::launch_batch_pass(L, st);
+    L.primes = primes;
+    L.lds = lds;
+    L.nseg = (int)ids.size();
+    long long end = 0;
+    for (size_t k = 0; k < ids.size(); k++) {
+      const Req &r = s->reqs[ids[k]];
+      const Plan &p = *r.plan;
+      char *base = (char *)p.dev;
+      end += p.ntiles[d];
+      L.seg[k] = dm::BatchSeg{(const dm::BatchTile *)(base + p.o_tiles[d]), (const dm::BatchSched *)(base + p.o_sched),
+                              base + p.o_tw, r.dev, end};
+    }
+    L.ntiles = end;
+    L.grid = std::min<long long>(end, resident);
+    ok = dm::launch_ok(nullptr, dm::launch_batch_pass(L, st), "dm_server: batched pass launch");
   }
+  ok = ok && hipEventRecord(sl.ev[2], st) == hipSuccess;
   for (long long id : ids) {
     const Req &r = s->reqs[id];
     ok = ok && hipMemcpyAsync(r.out, r.dev, r.bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
   }
-  ok = ok && hipEventRecord(sl.done, st) == hipSuccess;
-  double t = s->now();
+  ok = ok && hipEventRecord(sl.ev[3], st) == hipSuccess;
+  if (!ok) {
+    // the requests stay admitted (ready) with their buffers; the caller sees
+    // the failure through poll's -1
+    (void)hipStreamSynchronize(st);
+    raise_error(nullptr, "dm_server: a copy, launch or event call failed");
+    return false;
+  }
+  for (long long id : ids) s->ready.erase(std::find(s->ready.begin(), s->ready.end(), id));
+  const double t = s->now();
   for (long long id : ids) {
-    s->reqs[id].state = LAUNCHED;
-    s->reqs[id].t_launch = t;
+    Req &r = s->reqs[id];
+    r.state = LAUNCHED;
+    r.t_launch = t;
+    r.host_ms = (t - t_host) * 1e3;
   }
   sl.reqs = ids;
   sl.busy = true;
   s->batches++;
   s->max_seen = std::max<long long>(s->max_seen, (long long)ids.size());
-  if (!ok) raise_error(nullptr, "dm_server: a copy or event call failed");
-  return ok;
+  return true;
 }
 
 extern "C" int dm_server_poll(DmServer s) {
@@ -353,6 +450,19 @@ extern "C" int dm_server_times(DmServer s, long long id, double *submitted, doub
   return 0;
 }
 
+extern "C" int dm_server_breakdown(DmServer s, long long id, double *ms) {
+  if (s == nullptr || ms == nullptr || id < 0 || id >= (long long)s->reqs.size()) return 1;
+  const Req &r = s->reqs[id];
+  if (r.state != FINISHED) return 1;
+  ms[0] = (r.t_launch - r.t_sub) * 1e3 - r.host_ms;
+  ms[1] = r.host_ms;
+  ms[2] = r.h2d_ms;
+  ms[3] = r.kern_ms;
+  ms[4] = r.d2h_ms;
+  ms[5] = (r.t_done - r.t_launch) * 1e3 - (r.h2d_ms + r.kern_ms + r.d2h_ms);
+  return 0;
+}
+
 extern "C" int dm_server_stats(DmServer s, long long *batches, long long *max_batch_seen, size_t *peak_bytes,
                                size_t *pool_bytes) {
   if (s == nullptr) return 1;
@@ -363,20 +473,28 @@ extern "C" int dm_server_stats(DmServer s, long long *batches, long long *max_ba
   return 0;
 }
 
+extern "C" int dm_server_plan_stats(DmServer s, long long *cached, long long *builds, size_t *bytes) {
+  if (s == nullptr) return 1;
+  if (cached) *cached = (long long)s->plans.size();
+  if (builds) *builds = s->plan_builds;
+  if (bytes) *bytes = s->plan_bytes;
+  return 0;
+}
+
 extern "C" void dm_server_destroy(DmServer s) {
   if (s == nullptr) return;
   for (auto &sl : s->slots)
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
   for (auto &sl : s->slots) {
-    if (sl.done) (void)hipEventDestroy(sl.done);
+    for (auto &e : sl.ev)
+      if (e) (void)hipEventDestroy(e);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    dm::device_free(sl.arena);
-    if (sl.stage) (void)hipHostFree(sl.stage);
   }
   for (auto &r : s->reqs)
     if (r.dev) dm::device_free(r.dev);
   for (auto &kv : s->pool)
     for (void *p : kv.second) dm::device_free(p);
+  for (auto &kv : s->plans) plan_free(kv.second);
   delete s;
 }
 

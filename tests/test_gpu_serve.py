@@ -1,7 +1,10 @@
 """The request loop (dm_serve.h) on the device: every result against
 numpy's FP64 DFT (complex float within 1e-6, complex double within 1e-12
-relative L2), each result the same (bit for bit) alone and in a mix of
-other requests arriving while it runs, and the HBM budget respected."""
+relative L2) over the full axis range of the serving workload ([32, 256]
+in steps of 16), each result the same (bit for bit) alone and in a mix of
+other requests arriving while it runs, the HBM budget respected, plans
+built once per shape and reused, and the latency breakdown accounting for
+each request's time."""
 import ctypes
 
 import numpy as np
@@ -40,7 +43,7 @@ class Pinned:
         self.C.dm_host_free(self.ptr)
 
 
-def make_requests(C, n, seed, max_axis=96):
+def make_requests(C, n, seed, max_axis=256):
     rng = np.random.default_rng(seed)
     reqs = []
     for (_, shape, prec, direction) in workloads.serving_requests(n, 1000.0, seed=seed):
@@ -70,8 +73,9 @@ def release(reqs):
 
 
 def test_server_results_match_numpy(C):
-    s = C.dm_server_create(4 << 30, 4, 8)
-    reqs = make_requests(C, 40, seed=21)
+    s = C.dm_server_create(8 << 30, 4, 8)
+    reqs = make_requests(C, 32, seed=21)
+    assert max(max(r[0]) for r in reqs) >= 192, "the draw should reach the top of the axis range"
     for r in reqs:
         submit(C, s, r)
         C.dm_server_poll(s)
@@ -92,15 +96,15 @@ def test_server_alone_equals_mix(C):
     probe = make_requests(C, 5, seed=31)
     alone = []
     for r in probe:
-        s = C.dm_server_create(4 << 30, 1, 8)
+        s = C.dm_server_create(8 << 30, 1, 8)
         rid = submit(C, s, r)
         assert C.dm_server_wait(s, rid) == 0
         alone.append(r[5].array())
         C.dm_server_destroy(s)
     for r in probe:  # clear the outputs
         ctypes.memset(r[5].ptr, 0, r[3].nbytes)
-    others = make_requests(C, 30, seed=32)
-    s = C.dm_server_create(4 << 30, 3, 6)
+    others = make_requests(C, 24, seed=32)
+    s = C.dm_server_create(8 << 30, 3, 6)
     ids = []
     k = 0
     for i, r in enumerate(others):  # probes arrive while others are in flight
@@ -138,3 +142,60 @@ def test_server_budget(C):
         assert np.linalg.norm(got - want(r)) / np.linalg.norm(want(r)) < TOL[r[1]]
     C.dm_server_destroy(s)
     release(reqs)
+
+
+def test_server_plans_stay_on_the_device(C):
+    """Requests of one shape share one device plan: 12 requests over 3
+    shapes (both precisions and directions count as shapes of their own)
+    build 3 plans; every finished request has a latency breakdown whose
+    parts are non-negative and add up to its latency."""
+    s = C.dm_server_create(1 << 30, 2, 4)
+    rng = np.random.default_rng(51)
+    shapes = [(48, 64, 32), (256, 32, 32), (32, 32, 160)]
+    reqs = []
+    for i in range(12):
+        shape = shapes[i % 3]
+        x = workloads.uniform_complex(rng, shape, 0)
+        reqs.append((shape, 0, -1, x, Pinned(C, x), Pinned(C, np.zeros_like(x))))
+    ids = []
+    for r in reqs:
+        ids.append(submit(C, s, r))
+        C.dm_server_poll(s)
+    assert C.dm_server_drain(s) == 0
+    cached, builds = ctypes.c_longlong(), ctypes.c_longlong()
+    nbytes = ctypes.c_size_t()
+    assert C.dm_server_plan_stats(s, ctypes.byref(cached), ctypes.byref(builds), ctypes.byref(nbytes)) == 0
+    assert builds.value == 3 and cached.value == 3 and nbytes.value > 0
+    ms = (ctypes.c_double * 6)()
+    sub, fin = ctypes.c_double(), ctypes.c_double()
+    for rid, r in zip(ids, reqs):
+        assert C.dm_server_breakdown(s, rid, ms) == 0
+        parts = list(ms)
+        assert min(parts[:5]) >= 0 and parts[5] > -0.05, parts
+        C.dm_server_times(s, rid, ctypes.byref(sub), None, ctypes.byref(fin))
+        assert abs(sum(parts) - (fin.value - sub.value) * 1e3) < 1e-6
+        assert parts[3] > 0
+        got = r[5].array().astype(np.complex128)
+        assert np.linalg.norm(got - want(r)) / np.linalg.norm(want(r)) < TOL[0]
+    C.dm_server_destroy(s)
+    release(reqs)
+
+
+def test_server_refuses_without_abort(C):
+    """A request the server cannot take (an axis with a prime factor above
+    13) returns -1 with the reason in dm_last_error, in the default error
+    mode, and the server keeps serving."""
+    s = C.dm_server_create(1 << 30, 1, 2)
+    x = np.zeros((34, 32, 32), np.complex64)
+    pin = Pinned(C, x)
+    C.dm_clear_error()
+    assert C.dm_server_submit(s, 3, dm.ints([34, 32, 32]), 0, -1, pin.ptr, pin.ptr) == -1
+    assert b"not supported" in C.dm_last_error()
+    C.dm_clear_error()
+    y = workloads.uniform_complex(np.random.default_rng(3), (32, 32, 32), 1)
+    req = ((32, 32, 32), 1, -1, y, Pinned(C, y), Pinned(C, np.zeros_like(y)))
+    assert C.dm_server_wait(s, submit(C, s, req)) == 0
+    assert np.linalg.norm(req[5].array() - want(req)) / np.linalg.norm(want(req)) < TOL[1]
+    C.dm_server_destroy(s)
+    pin.free()
+    release([req])
