@@ -303,7 +303,7 @@ def serving_bench(C, dm, count=1000, streams=4, max_batch=8, budget=64 << 30):
     # and of those around the median; "arrival_lag" is the submit time after
     # the scheduled arrival (the driver loop's own delay)
     lag = np.array([(sub - first - arr) * 1e3 for (sub, _), arr in zip(times, arrivals)])
-    names = ["queue", "host_enqueue", "copy_in", "kernels", "copy_out", "completion_lag"]
+    names = ["queue", "host_enqueue_overlapped", "copy_in", "kernels", "copy_out", "completion_lag"]
 
     def mean_parts(sel):
         out = {n: round(float(parts[sel, i].mean()), 3) for i, n in enumerate(names)}

@@ -101,6 +101,40 @@ extern "C" DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int pre
 
 namespace dm {
 
+void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vector<double> *tw) {
+  BatchSched S{};
+  std::vector<int> rad;
+  choose_radices(N, &rad);
+  if (radix_order == DM_BATCH_RADIX_SMALL_FIRST) std::reverse(rad.begin(), rad.end());
+  S.N = N;
+  S.nstage = (int)rad.size();
+  S.T = T;
+  S.m_n = magic(N);
+  S.m_t = magic(T);
+  S.tw_off = 0;
+  int ns = 1;
+  std::vector<double> root = roots(N);
+  tw->assign(2 * (size_t)N, 0.0);
+  for (int s = 0; s < S.nstage; s++) {
+    int R = rad[s];
+    S.radix[s] = R;
+    S.st[s].nr = N / R;
+    S.st[s].ns = ns;
+    S.st[s].m_nr = magic(N / R);
+    S.st[s].m_ns = magic(ns);
+    int stride = N / (ns * R);
+    for (int r = 1; r < R; r++)
+      for (int jj = 0; jj < ns; jj++) {
+        size_t at = (size_t)(ns - 1) + (size_t)(r - 1) * ns + jj;
+        size_t k = (size_t)jj * r * stride;
+        (*tw)[2 * at] = root[2 * k];
+        (*tw)[2 * at + 1] = root[2 * k + 1];
+      }
+    ns *= R;
+  }
+  *out = S;
+}
+
 // Host image of a batch: tiles per axis pass (last axis first), schedules
 // and their stage-ordered twiddles (complex double), LDS need, and whether a
 // radix 5, 7, 11 or 13 occurs.
@@ -115,36 +149,11 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
     auto it = sched_of.find(key);
     if (it != sched_of.end()) return it->second;
     BatchSched S{};
-    std::vector<int> rad;
-    choose_radices(N, &rad);
-    if (radix_order == DM_BATCH_RADIX_SMALL_FIRST) std::reverse(rad.begin(), rad.end());
-    S.N = N;
-    S.nstage = (int)rad.size();
-    S.T = T;
-    S.m_n = magic(N);
-    S.m_t = magic(T);
-    int ns = 1;
+    std::vector<double> block;
+    make_batch_sched(N, T, radix_order, &S, &block);
     S.tw_off = (int)(h.tw.size() / 2);
-    std::vector<double> root = roots(N);
-    std::vector<double> block(2 * (size_t)N, 0.0);
-    for (int s = 0; s < S.nstage; s++) {
-      int R = rad[s];
-      S.radix[s] = R;
-      if (R == 5 || R == 7 || R == 11 || R == 13) h.primes = 1;
-      S.st[s].nr = N / R;
-      S.st[s].ns = ns;
-      S.st[s].m_nr = magic(N / R);
-      S.st[s].m_ns = magic(ns);
-      int stride = N / (ns * R);
-      for (int r = 1; r < R; r++)
-        for (int jj = 0; jj < ns; jj++) {
-          size_t at = (size_t)(ns - 1) + (size_t)(r - 1) * ns + jj;
-          size_t k = (size_t)jj * r * stride;
-          block[2 * at] = root[2 * k];
-          block[2 * at + 1] = root[2 * k + 1];
-        }
-      ns *= R;
-    }
+    for (int s = 0; s < S.nstage; s++)
+      if (S.radix[s] == 5 || S.radix[s] == 7 || S.radix[s] == 11 || S.radix[s] == 13) h.primes = 1;
     h.tw.insert(h.tw.end(), block.begin(), block.end());
     int id = (int)h.sched.size();
     h.sched.push_back(S);

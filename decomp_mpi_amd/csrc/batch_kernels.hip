@@ -131,11 +131,27 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     if (p.nseg > 0) {
       int sg = 0;
       while (sg + 1 < p.nseg && t >= p.seg[sg].end) sg++;
-      const long long first = sg > 0 ? p.seg[sg - 1].end : 0;
-      d = p.seg[sg].tiles[t - first];
-      sched = p.seg[sg].sched;
-      twall = reinterpret_cast<const cx<R> *>(p.seg[sg].twiddle);
-      xdata = p.seg[sg].data;
+      const BatchSeg &g = p.seg[sg];
+      const long long q = t - (sg > 0 ? p.seg[sg - 1].end : 0);
+      if (g.inner == 1) {  // rows: T consecutive lines of N contiguous points
+        const long long o = q * g.T;
+        d.base = o * g.N;
+        d.ps = 1;
+        d.nl = (int)min((long long)g.T, (long long)g.outer - o);
+        d.row = 1;
+      } else {             // columns: T consecutive inner offsets of one outer coordinate
+        const long long o = q / g.ctiles;
+        const long long c = (q - o * g.ctiles) * g.T;
+        d.base = o * g.N * g.inner + c;
+        d.ps = g.inner;
+        d.nl = (int)min((long long)g.T, g.inner - c);
+        d.row = 0;
+      }
+      d.xform = 0;
+      d.sched = 0;
+      sched = g.sched;
+      twall = reinterpret_cast<const cx<R> *>(g.twiddle);
+      xdata = g.data;
     } else {
       d = p.tiles[t];
       sched = p.sched;

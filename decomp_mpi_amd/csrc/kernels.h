@@ -116,15 +116,24 @@ struct BatchSched {
   unsigned long long m_n, m_t;
   StageInfo st[16];
 };
-// A segment: the tiles of one transform whose tables live apart from the
-// others' (a device-resident per-shape plan of the request loop).  Its
-// tiles have xform 0 and address that transform's data directly.
+// A segment: the pass over one axis of one transform, described by its
+// shape instead of a tile list (the request loop's launches).  The lines of
+// N points have point stride `inner`.  Rows (inner 1): `outer` lines of N
+// contiguous points, tiles of T consecutive lines.  Columns: for each of
+// `outer` coordinates, inner lines at consecutive offsets, `ctiles` tiles of
+// T of them.  Tile q of the segment is computed, not looked up.  The
+// schedule (one BatchSched, roots at tw_off 0 of `twiddle`) is a device-
+// resident entry shared by every transform with that line length and tile
+// height.
 struct BatchSeg {
-  const BatchTile *tiles;
   const BatchSched *sched;
   const void *twiddle;
   void *data;
   long long end;               // launch tiles [previous end, end) are this segment's
+  long long inner;             // point stride of the axis
+  int outer;                   // rows: lines; columns: outer coordinates
+  int ctiles;                  // columns: tiles per outer coordinate
+  int N, T;                    // the schedule's line length and tile height
 };
 #define DM_BATCH_MAX_SEG 16
 struct BatchLaunch {
@@ -137,7 +146,7 @@ struct BatchLaunch {
   int primes;                  // some schedule has a radix 5, 7, 11 or 13
   long long grid;              // persistent workgroups (tile loop inside)
   size_t lds;
-  int nseg;                    // > 0: tiles come from seg[] (tiles/sched/data/twiddle unused)
+  int nseg;                    // > 0: tiles come from seg[] (tiles, sched, data, twiddle unused)
   BatchSeg seg[DM_BATCH_MAX_SEG];
 };
 int launch_batch_pass(const BatchLaunch &p, void *stream);
@@ -153,6 +162,9 @@ struct BatchHost {
 };
 bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_choice, int radix_order,
                       BatchHost *out);
+// One schedule: lines of N points (radices 2..16, 3, 5, 7, 11, 13) in tiles
+// of T lines, tw_off 0; tw: its N stage-ordered roots as (re, im) doubles.
+void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vector<double> *tw);
 int batch_tile_capacity(int dbl);
 size_t batch_lds_bytes(int dbl, int T, int N);
 long long batch_resident_groups(int dbl, int primes, size_t lds);

@@ -8,15 +8,17 @@
 // request plus the following admitted requests of the same precision,
 // direction and rank, up to max_batch.
 //
-// Plans are device-resident: the first request of a (shape, precision,
-// direction) builds that shape's tables once -- radix schedules, their
-// twiddles and the tile list of every axis pass -- and uploads them; later
-// requests of the shape reuse them (least recently used plans go when the
-// cache is full).  A batch is then: input copies, one batched pass launch
-// per axis whose kernel argument lists each request's tables and data as a
-// segment (nothing is rebuilt or uploaded per batch), output copies.  Four
-// timing events per batch give each request's copy-in, kernel and copy-out
-// time beside its queue and host time.
+// Plans are device-resident: a transform's plan is, per axis, a radix
+// schedule with its twiddles for the line length and tile height of that
+// axis; the first request needing a (length, tile height, precision) builds
+// and uploads that schedule once and every later request reuses it (least
+// recently used entries go when the cache is full).  Tiles are not listed:
+// the kernel computes a tile's lines from the transform's shape.  A batch is
+// then: input copies, one batched pass launch per axis whose kernel argument
+// lists each request's schedule, shape and data as a segment (nothing is
+// built or uploaded per batch), output copies.  Four timing events per batch
+// give each request's copy-in, kernel and copy-out time beside its queue and
+// host time.
 //
 // A request's transform depends only on its own shape (tile heights and
 // radix schedules are functions of the line length), so its result is the
@@ -46,18 +48,28 @@ using Clock = std::chrono::steady_clock;
 
 enum State { QUEUED, ADMITTED, LAUNCHED, FINISHED };
 
-// Tables of one shape on the device: schedules | roots | tiles of each pass.
-struct Plan {
+// One radix schedule on the device: its BatchSched (tw_off 0) and roots.
+struct Sched {
   void *dev = nullptr;
   size_t bytes = 0;
-  size_t o_sched = 0, o_tw = 0, o_tiles[3] = {0, 0, 0};
+  const dm::BatchSched *sched = nullptr;
+  const void *tw = nullptr;
+  size_t lds = 0;
+  int primes = 0;
+  int refs = 0;             // requests admitted with this schedule and not yet finished
+  long long last_use = 0;
+};
+typedef std::tuple<int, int, int> SchedKey;  // N, T, dbl
+
+// A request's plan: one segment (less data and tile end) per axis pass.
+struct ReqPlan {
+  int npass = 0;
+  Sched *sched[3] = {nullptr, nullptr, nullptr};
+  dm::BatchSeg seg[3];
   long long ntiles[3] = {0, 0, 0};
   size_t lds = 0;
   int primes = 0;
-  int refs = 0;             // requests admitted with this plan and not yet finished
-  long long last_use = 0;
 };
-typedef std::tuple<int, int, int, int, int, int> PlanKey;  // ndim, shape[3], dbl, inverse
 
 struct Req {
   int ndim = 0;
@@ -67,7 +79,8 @@ struct Req {
   void *out = nullptr;
   size_t bytes = 0, cls = 0;
   void *dev = nullptr;
-  Plan *plan = nullptr;
+  ReqPlan plan;
+  bool planned = false;
   State state = QUEUED;
   double t_sub = 0, t_launch = 0, t_done = 0;
   double host_ms = 0, h2d_ms = 0, kern_ms = 0, d2h_ms = 0;
@@ -104,8 +117,8 @@ void reject(const char *fmt, ...) {
   raise_error(nullptr, "%s", msg);
 }
 
-constexpr size_t kPlanCacheEntries = 256;
-constexpr size_t kPlanCacheBytes = (size_t)256 << 20;
+constexpr size_t kSchedCacheEntries = 1024;
+constexpr size_t kSchedCacheBytes = (size_t)64 << 20;
 
 }  // namespace
 
@@ -116,9 +129,9 @@ struct dm_server {
   std::vector<Req> reqs;
   std::deque<long long> queue, ready;
   std::map<size_t, std::vector<void *>> pool;  // free device buffers by class
-  std::map<PlanKey, Plan *> plans;
-  size_t plan_bytes = 0;
-  long long plan_builds = 0, plan_clock = 0;
+  std::map<SchedKey, Sched *> scheds;
+  size_t sched_bytes = 0;
+  long long sched_builds = 0, sched_clock = 0;
   Clock::time_point t0 = Clock::now();
   long long batches = 0, max_seen = 0;
   long long unfinished = 0;
@@ -148,79 +161,125 @@ static void *pool_get(DmServer s, size_t cls) {
 
 static void pool_put(DmServer s, void *p, size_t cls) { s->pool[cls].push_back(p); }
 
-static void plan_free(Plan *p) {
+static void sched_free(Sched *p) {
   dm::device_free(p->dev);
   delete p;
 }
 
-// Drop least recently used plans that no request holds until the cache is
-// within its entry and byte limits.
-static void plan_trim(DmServer s) {
-  while (s->plans.size() > kPlanCacheEntries || s->plan_bytes > kPlanCacheBytes) {
-    auto victim = s->plans.end();
-    for (auto it = s->plans.begin(); it != s->plans.end(); ++it)
-      if (it->second->refs == 0 && (victim == s->plans.end() || it->second->last_use < victim->second->last_use))
+// Drop least recently used schedules that no request holds until the cache
+// is within its entry and byte limits.
+static void sched_trim(DmServer s) {
+  while (s->scheds.size() > kSchedCacheEntries || s->sched_bytes > kSchedCacheBytes) {
+    auto victim = s->scheds.end();
+    for (auto it = s->scheds.begin(); it != s->scheds.end(); ++it)
+      if (it->second->refs == 0 && (victim == s->scheds.end() || it->second->last_use < victim->second->last_use))
         victim = it;
-    if (victim == s->plans.end()) return;  // every cached plan is in use
-    s->plan_bytes -= victim->second->bytes;
-    plan_free(victim->second);
-    s->plans.erase(victim);
+    if (victim == s->scheds.end()) return;  // every cached schedule is in use
+    s->sched_bytes -= victim->second->bytes;
+    sched_free(victim->second);
+    s->scheds.erase(victim);
   }
 }
 
-// The device tables of a request's shape, with a reference taken for the
-// caller: from the cache, or built and uploaded once.  Null (after an
-// error message) if the device refuses.
-static Plan *plan_for(DmServer s, const Req &r) {
-  PlanKey key(r.ndim, r.shape[0], r.shape[1], r.shape[2], r.dbl, r.inv);
-  auto it = s->plans.find(key);
-  if (it != s->plans.end()) {
-    it->second->last_use = ++s->plan_clock;
+// The device schedule of lines of N points in tiles of T lines, with a
+// reference taken for the caller: from the cache, or built and uploaded
+// once.  Null (after an error message) if the device refuses.
+static Sched *sched_for(DmServer s, int N, int T, int dbl) {
+  SchedKey key(N, T, dbl);
+  auto it = s->scheds.find(key);
+  if (it != s->scheds.end()) {
+    it->second->last_use = ++s->sched_clock;
     it->second->refs++;
     return it->second;
   }
-  dm::BatchHost h;
-  if (!dm::build_batch_host(1, r.ndim, r.shape, r.dbl, DM_BATCH_TILE_FULL, DM_BATCH_RADIX_LARGE_FIRST, &h)) {
-    raise_error(nullptr, "dm_server: cannot plan a transform of this shape");
-    return nullptr;
-  }
-  Plan *p = new Plan;
-  p->o_sched = 0;
-  p->o_tw = align256(h.sched.size() * sizeof(dm::BatchSched));
-  const size_t tw_bytes = (h.tw.size() / 2) * (r.dbl ? 16 : 8);
-  size_t at = p->o_tw + align256(tw_bytes);
-  for (int d = 0; d < r.ndim; d++) {
-    p->o_tiles[d] = at;
-    p->ntiles[d] = (long long)h.tiles[d].size();
-    at += align256(h.tiles[d].size() * sizeof(dm::BatchTile));
-  }
-  std::vector<char> img(at, 0);
-  memcpy(img.data() + p->o_sched, h.sched.data(), h.sched.size() * sizeof(dm::BatchSched));
-  if (r.dbl) {
-    memcpy(img.data() + p->o_tw, h.tw.data(), tw_bytes);
+  // the schedule build_batch_host gives lines of N points in tiles of T
+  dm::BatchSched one{};
+  std::vector<double> tw;
+  dm::make_batch_sched(N, T, DM_BATCH_RADIX_LARGE_FIRST, &one, &tw);
+  const size_t tw_bytes = (size_t)N * (dbl ? 16 : 8);
+  const size_t o_tw = align256(sizeof one);
+  std::vector<char> img(o_tw + tw_bytes, 0);
+  memcpy(img.data(), &one, sizeof one);
+  if (dbl) {
+    memcpy(img.data() + o_tw, tw.data(), tw_bytes);
   } else {
-    float *f = (float *)(img.data() + p->o_tw);
-    for (size_t i = 0; i < h.tw.size(); i++) f[i] = (float)h.tw[i];
+    float *f = (float *)(img.data() + o_tw);
+    for (size_t i = 0; i < 2 * (size_t)N; i++) f[i] = (float)tw[i];
   }
-  for (int d = 0; d < r.ndim; d++)
-    memcpy(img.data() + p->o_tiles[d], h.tiles[d].data(), h.tiles[d].size() * sizeof(dm::BatchTile));
-  p->dev = dm::device_alloc(at);
-  if (p->dev == nullptr || hipMemcpy(p->dev, img.data(), at, hipMemcpyHostToDevice) != hipSuccess) {
+  Sched *p = new Sched;
+  p->bytes = img.size();
+  p->dev = dm::device_alloc(p->bytes);
+  if (p->dev == nullptr || hipMemcpy(p->dev, img.data(), p->bytes, hipMemcpyHostToDevice) != hipSuccess) {
     dm::device_free(p->dev);
     delete p;
-    raise_error(nullptr, "dm_server: device allocation or upload of a plan (%zu bytes) failed", at);
+    raise_error(nullptr, "dm_server: device allocation or upload of a schedule failed");
     return nullptr;
   }
-  p->bytes = at;
-  p->lds = h.lds;
-  p->primes = h.primes;
-  p->last_use = ++s->plan_clock;
+  p->sched = (const dm::BatchSched *)p->dev;
+  p->tw = (char *)p->dev + o_tw;
+  p->lds = dm::batch_lds_bytes(dbl, T, N);
+  for (int k = 0; k < one.nstage; k++)
+    if (one.radix[k] == 5 || one.radix[k] == 7 || one.radix[k] == 11 || one.radix[k] == 13) p->primes = 1;
+  p->last_use = ++s->sched_clock;
   p->refs = 1;  // held by the caller, so the trim below keeps it
-  s->plans[key] = p;
-  s->plan_bytes += at;
-  s->plan_builds++;
-  plan_trim(s);
+  s->scheds[key] = p;
+  s->sched_bytes += p->bytes;
+  s->sched_builds++;
+  sched_trim(s);
   return p;
+}
+
+static void plan_release(DmServer s, Req &r) {
+  for (int d = 0; d < r.plan.npass; d++)
+    if (r.plan.sched[d]) r.plan.sched[d]->refs--;
+  r.plan = ReqPlan();
+  r.planned = false;
+  (void)s;
+}
+
+// A request's plan: for the pass over each axis (last axis first, as the
+// batched transforms run), the schedule and the segment geometry, with the
+// tile height rule of build_batch_host (as many lines as the tile holds; a
+// column walk never more than its contiguous run).
+static bool plan_request(DmServer s, Req &r) {
+  const int cap = dm::batch_tile_capacity(r.dbl);
+  ReqPlan &pl = r.plan;
+  pl = ReqPlan();
+  pl.npass = r.ndim;
+  for (int k = 0; k < r.ndim; k++) {
+    const int d = r.ndim - 1 - k;
+    long long inner = 1, outer = 1;
+    for (int e = d + 1; e < r.ndim; e++) inner *= r.shape[e];
+    for (int e = 0; e < d; e++) outer *= r.shape[e];
+    const int N = r.shape[d];
+    int T = std::max(1, cap / N);
+    if (inner > 1) T = (int)std::min<long long>(T, inner);
+    Sched *sc = sched_for(s, N, T, r.dbl);
+    if (sc == nullptr) {
+      plan_release(s, r);
+      return false;
+    }
+    pl.sched[k] = sc;
+    dm::BatchSeg &g = pl.seg[k];
+    g = dm::BatchSeg{};
+    g.sched = sc->sched;
+    g.twiddle = sc->tw;
+    g.inner = inner;
+    g.outer = (int)outer;
+    g.N = N;
+    g.T = T;
+    if (inner == 1) {
+      g.ctiles = 0;
+      pl.ntiles[k] = (outer + T - 1) / T;
+    } else {
+      g.ctiles = (int)((inner + T - 1) / T);
+      pl.ntiles[k] = outer * g.ctiles;
+    }
+    pl.lds = std::max(pl.lds, sc->lds);
+    pl.primes = pl.primes || sc->primes;
+  }
+  r.planned = true;
+  return true;
 }
 
 extern "C" DmServer dm_server_create(size_t budget, int streams, int max_batch) {
@@ -308,29 +367,26 @@ static void retire(DmServer s) {
       r.d2h_ms = d2h;
       pool_put(s, r.dev, r.cls);
       r.dev = nullptr;
-      r.plan->refs--;
-      r.plan = nullptr;
+      plan_release(s, r);
       s->in_use -= r.cls;
       s->unfinished--;
     }
     sl.reqs.clear();
     sl.busy = false;
   }
-  plan_trim(s);
+  sched_trim(s);
 }
 
 static void admit(DmServer s) {
   while (!s->queue.empty()) {
     Req &r = s->reqs[s->queue.front()];
     if (s->in_use + r.cls > s->budget) break;  // arrival order: no overtaking
-    Plan *p = plan_for(s, r);
-    if (p == nullptr) break;
+    if (!plan_request(s, r)) break;
     r.dev = pool_get(s, r.cls);
     if (r.dev == nullptr) {  // device full: wait for buffers to come back
-      p->refs--;
+      plan_release(s, r);
       break;
     }
-    r.plan = p;
     s->in_use += r.cls;
     s->peak = std::max(s->peak, s->in_use);
     r.state = ADMITTED;
@@ -360,8 +416,8 @@ static bool launch(DmServer s, Slot &sl) {
   size_t lds = 0;
   int primes = 0;
   for (long long id : ids) {
-    lds = std::max(lds, s->reqs[id].plan->lds);
-    primes = primes || s->reqs[id].plan->primes;
+    lds = std::max(lds, s->reqs[id].plan.lds);
+    primes = primes || s->reqs[id].plan.primes;
   }
   const long long resident = dm::batch_resident_groups(dbl, primes, lds);
   for (int d = 0; d < nd && ok; d++) {
@@ -374,11 +430,10 @@ static bool launch(DmServer s, Slot &sl) {
     long long end = 0;
     for (size_t k = 0; k < ids.size(); k++) {
       const Req &r = s->reqs[ids[k]];
-      const Plan &p = *r.plan;
-      char *base = (char *)p.dev;
-      end += p.ntiles[d];
-      L.seg[k] = dm::BatchSeg{(const dm::BatchTile *)(base + p.o_tiles[d]), (const dm::BatchSched *)(base + p.o_sched),
-                              base + p.o_tw, r.dev, end};
+      end += r.plan.ntiles[d];
+      L.seg[k] = r.plan.seg[d];
+      L.seg[k].data = r.dev;
+      L.seg[k].end = end;
     }
     L.ntiles = end;
     L.grid = std::min<long long>(end, resident);
@@ -402,7 +457,7 @@ static bool launch(DmServer s, Slot &sl) {
   for (long long id : ids) {
     Req &r = s->reqs[id];
     r.state = LAUNCHED;
-    r.t_launch = t;
+    r.t_launch = t_host;  // the batch's first event is recorded right after
     r.host_ms = (t - t_host) * 1e3;
   }
   sl.reqs = ids;
@@ -454,7 +509,7 @@ extern "C" int dm_server_breakdown(DmServer s, long long id, double *ms) {
   if (s == nullptr || ms == nullptr || id < 0 || id >= (long long)s->reqs.size()) return 1;
   const Req &r = s->reqs[id];
   if (r.state != FINISHED) return 1;
-  ms[0] = (r.t_launch - r.t_sub) * 1e3 - r.host_ms;
+  ms[0] = (r.t_launch - r.t_sub) * 1e3;
   ms[1] = r.host_ms;
   ms[2] = r.h2d_ms;
   ms[3] = r.kern_ms;
@@ -475,9 +530,9 @@ extern "C" int dm_server_stats(DmServer s, long long *batches, long long *max_ba
 
 extern "C" int dm_server_plan_stats(DmServer s, long long *cached, long long *builds, size_t *bytes) {
   if (s == nullptr) return 1;
-  if (cached) *cached = (long long)s->plans.size();
-  if (builds) *builds = s->plan_builds;
-  if (bytes) *bytes = s->plan_bytes;
+  if (cached) *cached = (long long)s->scheds.size();
+  if (builds) *builds = s->sched_builds;
+  if (bytes) *bytes = s->sched_bytes;
   return 0;
 }
 
@@ -494,7 +549,7 @@ extern "C" void dm_server_destroy(DmServer s) {
     if (r.dev) dm::device_free(r.dev);
   for (auto &kv : s->pool)
     for (void *p : kv.second) dm::device_free(p);
-  for (auto &kv : s->plans) plan_free(kv.second);
+  for (auto &kv : s->scheds) sched_free(kv.second);
   delete s;
 }
 

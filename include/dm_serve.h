@@ -50,12 +50,13 @@ int dm_server_drain(DmServer s);
  *  submitted, launched on the device and seen finished; 0 until reached. */
 int dm_server_times(DmServer s, long long id, double *submitted, double *launched, double *finished);
 /** Where a finished request's latency went, in milliseconds: ms[0] queue
- *  (submitted until its batch was enqueued, less ms[1]), ms[1] host time to
- *  enqueue its batch, ms[2] input copies, ms[3] kernels, ms[4] output
- *  copies (ms[2..4] are its batch's, from events on the batch's stream),
- *  ms[5] the rest until a poll saw it finished (the stream starting late
- *  behind other streams' work, and poll spacing).  Returns 1 until the
- *  request has finished. */
+ *  (submitted until its batch started to be enqueued), ms[2] input copies,
+ *  ms[3] kernels, ms[4] output copies (ms[2..4] are its batch's, from events
+ *  on the batch's stream), ms[5] the rest until a poll saw it finished (the
+ *  stream starting late behind other streams' work, and poll spacing);
+ *  ms[0] + ms[2] + ms[3] + ms[4] + ms[5] is the latency.  ms[1] is the host
+ *  time spent enqueueing the batch, which overlaps ms[2..4].  Returns 1
+ *  until the request has finished. */
 int dm_server_breakdown(DmServer s, long long id, double *ms);
 /** Counters: batches launched, largest batch, peak device bytes in use,
  *  device bytes held by the pool. */

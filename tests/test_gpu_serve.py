@@ -110,9 +110,10 @@ def test_server_alone_equals_mix(C):
     for i, r in enumerate(others):  # probes arrive while others are in flight
         submit(C, s, r)
         C.dm_server_poll(s)
-        if i % 6 == 3 and k < len(probe):
+        if i % 5 == 3 and k < len(probe):
             ids.append(submit(C, s, probe[k]))
             k += 1
+    assert k == len(probe)
     assert C.dm_server_drain(s) == 0
     for r, a in zip(probe, alone):
         assert np.array_equal(r[5].array(), a)
@@ -145,40 +146,46 @@ def test_server_budget(C):
 
 
 def test_server_plans_stay_on_the_device(C):
-    """Requests of one shape share one device plan: 12 requests over 3
-    shapes (both precisions and directions count as shapes of their own)
-    build 3 plans; every finished request has a latency breakdown whose
-    parts are non-negative and add up to its latency."""
+    """Plans are device-resident: 12 requests over 3 shapes build the
+    schedules their axes need (one per line length and tile height), and
+    12 more requests of the same shapes build none; every finished request
+    has a latency breakdown whose parts are non-negative and add up to its
+    latency (queue, copies, kernels, completion lag; the host enqueue time
+    overlaps the device parts)."""
     s = C.dm_server_create(1 << 30, 2, 4)
     rng = np.random.default_rng(51)
     shapes = [(48, 64, 32), (256, 32, 32), (32, 32, 160)]
-    reqs = []
-    for i in range(12):
-        shape = shapes[i % 3]
-        x = workloads.uniform_complex(rng, shape, 0)
-        reqs.append((shape, 0, -1, x, Pinned(C, x), Pinned(C, np.zeros_like(x))))
-    ids = []
-    for r in reqs:
-        ids.append(submit(C, s, r))
-        C.dm_server_poll(s)
-    assert C.dm_server_drain(s) == 0
     cached, builds = ctypes.c_longlong(), ctypes.c_longlong()
     nbytes = ctypes.c_size_t()
-    assert C.dm_server_plan_stats(s, ctypes.byref(cached), ctypes.byref(builds), ctypes.byref(nbytes)) == 0
-    assert builds.value == 3 and cached.value == 3 and nbytes.value > 0
     ms = (ctypes.c_double * 6)()
     sub, fin = ctypes.c_double(), ctypes.c_double()
-    for rid, r in zip(ids, reqs):
-        assert C.dm_server_breakdown(s, rid, ms) == 0
-        parts = list(ms)
-        assert min(parts[:5]) >= 0 and parts[5] > -0.05, parts
-        C.dm_server_times(s, rid, ctypes.byref(sub), None, ctypes.byref(fin))
-        assert abs(sum(parts) - (fin.value - sub.value) * 1e3) < 1e-6
-        assert parts[3] > 0
-        got = r[5].array().astype(np.complex128)
-        assert np.linalg.norm(got - want(r)) / np.linalg.norm(want(r)) < TOL[0]
+    seen = []
+    for rnd in range(2):
+        reqs, ids = [], []
+        for i in range(12):
+            shape = shapes[i % 3]
+            x = workloads.uniform_complex(rng, shape, 0)
+            reqs.append((shape, 0, -1, x, Pinned(C, x), Pinned(C, np.zeros_like(x))))
+        for r in reqs:
+            ids.append(submit(C, s, r))
+            C.dm_server_poll(s)
+        assert C.dm_server_drain(s) == 0
+        assert C.dm_server_plan_stats(s, ctypes.byref(cached), ctypes.byref(builds), ctypes.byref(nbytes)) == 0
+        seen.append((builds.value, cached.value))
+        assert 3 <= builds.value <= 9 and cached.value == builds.value and nbytes.value > 0
+        for rid, r in zip(ids, reqs):
+            assert C.dm_server_breakdown(s, rid, ms) == 0
+            parts = list(ms)
+            assert min(parts[:5]) >= 0 and parts[5] > -0.02, parts
+            C.dm_server_times(s, rid, ctypes.byref(sub), None, ctypes.byref(fin))
+            # ms[1] (host enqueue) overlaps the device parts
+            assert abs(sum(parts) - parts[1] - (fin.value - sub.value) * 1e3) < 1e-6
+            assert parts[3] > 0
+            got = r[5].array().astype(np.complex128)
+            assert np.linalg.norm(got - want(r)) / np.linalg.norm(want(r)) < TOL[0]
+        release(reqs)
+    assert seen[1] == seen[0], seen
     C.dm_server_destroy(s)
-    release(reqs)
 
 
 def test_server_refuses_without_abort(C):
