@@ -133,6 +133,7 @@ _SIGS = {
     "dm_batch_plan": (P, [I, I, IP, I, I]), "dm_batch_bind": (I, [P, ctypes.POINTER(P)]),
     "dm_batch_set_schedule": (I, [P, I, I]), "dm_batch_run": (I, [P, P]),
     "dm_batch_graph_launch": (I, [P, P]), "dm_batch_autotune": (D, [P, P]),
+    "dm_batch_run_noalloc": (I, [I, I, IP, I, I, ctypes.POINTER(P), P]),
     "dm_batch_info": (I, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), IP, IP]),
     "dm_batch_bytes": (D, [P]), "dm_batch_free": (None, [P]),
     # request loop (dm_serve.h)

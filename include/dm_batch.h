@@ -50,6 +50,25 @@ int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice
 double dm_batch_bytes(DmBatch b);
 void dm_batch_free(DmBatch b);
 
+/** Device bytes of the fixed table area that dm_batch_run_noalloc uses. */
+#define DM_BATCH_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+/** No counterpart in the reference.  For callers that cannot allocate
+ *  device memory (code under HIP graph capture, tight serving loops,
+ *  harnesses that forbid hipMalloc): plan, bind and run in one call that
+ *  allocates no device memory.  The tables the bound form allocates (data
+ *  pointers, schedules, twiddles, tile lists) go into a fixed
+ *  DM_BATCH_NOALLOC_SCRATCH_BYTES area that each device holds once for the
+ *  life of the process; a batch whose tables do not fit is refused (1,
+ *  after an error message).  Arguments as dm_batch_plan, plus data[i] (device
+ *  pointer of transform i, in place) and the stream (NULL: the legacy
+ *  default stream).  Returns 0 once the tables are on the device and the
+ *  passes are enqueued; the result equals dm_batch_run's bit for bit (full
+ *  tiles, large radices first).  Every call on one device shares its table
+ *  area: calls that overlap in time on one device (different streams or
+ *  host threads) race; calls on one stream are ordered by it. */
+int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int precision, int direction,
+                         void *const *data, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
