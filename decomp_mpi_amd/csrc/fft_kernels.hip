@@ -54,7 +54,7 @@ int launch_fft_pass(const FftLaunch &p, void *stream) {
     // Fused first/last stages need whole butterflies per thread (points per
     // thread a multiple of the radix) and an unpermuted walk along the line
     // in row mode (a permuted one goes through LDS to stay coalesced).
-    const int V = (p.dbl ? CAP_D : CAP_F) / NTH_BIG;
+    const int V = p.dbl ? CAP_D / NTH_BIG : CAP_F / f32_threads();
     auto pow2 = [](int r) { return r == 2 || r == 4 || r == 8 || r == 16; };
     int r0 = p.radix[0], rl = p.radix[p.nstage > 0 ? p.nstage - 1 : 0];
     P.fuse_in = p.nstage >= 1 && pow2(r0) && V % r0 == 0 && (!p.in_row || p.ord_in == nullptr);

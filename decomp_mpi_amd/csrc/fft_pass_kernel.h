@@ -251,8 +251,10 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
 // back, then the LDS side -- because vector memory operations complete in
 // order: a lookup issued after a data load would make the data load's
 // latency part of every later address computation.
+// Complex float at 8 points per thread is held to 64 VGPRs (8 waves per
+// SIMD: two 1024-thread or four 512-thread workgroups per CU).
 template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT>
-__global__ __launch_bounds__(NTH) void fft_pass_kernel(Pass P) {
+__global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) void fft_pass_kernel(Pass P) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   const FftLaunch &p = P.p;
@@ -386,6 +388,16 @@ inline unsigned long long magic_for(int d) { return (0x100000000ULL / (unsigned 
 constexpr int CAP_D = 4096;  // complex double points per big tile
 constexpr int CAP_F = 8192;  // complex float points per big tile
 constexpr int NTH_BIG = 512;
+// Threads of a big complex-float tile: 1024 (8 points per thread, as for
+// complex double) unless DM_F32_THREADS=512 (16 points per thread).
+inline int f32_threads() {
+  static int n = -1;
+  if (n < 0) {
+    const char *e = getenv("DM_F32_THREADS");
+    n = (e != nullptr && atoi(e) == 512) ? 512 : 1024;
+  }
+  return n;
+}
 
 template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT>
 hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
@@ -443,9 +455,17 @@ template <typename R, bool INV, bool POW2>
 hipError_t launch_tiles(const Pass &P, size_t lds, hipStream_t s) {
   constexpr int CAP = sizeof(R) == 8 ? CAP_D : CAP_F;
   const bool small = P.p.T * P.p.N <= CAP / 2;
+  if constexpr (sizeof(R) == 4) {
+    if (f32_threads() == 1024) {
+      if (small) return launch_layouts<R, INV, CAP / 2, NTH_BIG, POW2>(P, lds, s);
+      return launch_layouts<R, INV, CAP, 2 * NTH_BIG, POW2>(P, lds, s);
+    }
+  }
   if constexpr (sizeof(R) == 8) {
     // wide complex-double tile: 16 lines (256-byte column rows) on 1024
-    // threads, for column walks at a large line stride
+    // threads, for column walks at a large line stride (for complex float
+    // the same 256-byte rows measured slower: 2.66 vs 2.58 ms per 512^3
+    // step)
     if (P.p.T * P.p.N > CAP) return launch_layouts<R, INV, 2 * CAP, 2 * NTH_BIG, POW2>(P, lds, s);
   }
   if (small) return launch_layouts<R, INV, CAP / 2, NTH_BIG / 2, POW2>(P, lds, s);

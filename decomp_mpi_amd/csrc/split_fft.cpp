@@ -260,10 +260,16 @@ static bool dense_layout(const std::vector<int> &L, const std::vector<int> &orde
   return true;
 }
 
-// Build one direction's passes.  Forward: dat (start) -> W0 -> ... -> W0
-// (output); backward: W0 -> ... -> W0 -> dat.  The intermediate passes work
-// in place: every tile reads all of its points before writing them, tiles
-// are disjoint, and an intermediate pass reads and writes the same layout.
+// Build one direction's passes.  Forward: dat (start) -> ... -> W0
+// (output); backward: W0 (output) -> ... -> dat (start).  Each direction
+// orders its own passes the same way: the axis fastest in memory on the
+// side it reads first, then the others from fast to slow on the side it
+// writes last.  With dense intermediates (below) every pass then reads
+// whole contiguous lines and writes rows of T elements at the stride of a
+// line: the forward's and the backward's passes have the same access
+// pattern.  Without them, the intermediates stay in the output layout and
+// run in place (every tile reads all of its points before writing them,
+// tiles are disjoint, and such a pass reads and writes one layout).
 static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, const dm::Side &O, void *dat,
                               void *w0, void *w1, dm_fft_program *prog) {
   dm_split_ext *x = plan->ext;
@@ -272,46 +278,57 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
   for (int d = 0; d < nd; d++)
     if (x->L[d] > 1) axes.push_back(d);
   if (axes.empty()) axes.push_back(0);
-  // forward: start's fastest axis first, then the others from fast to slow
-  int f = S.h.fastest;
-  std::vector<int> order;
-  if (std::find(axes.begin(), axes.end(), f) != axes.end()) order.push_back(f);
-  std::vector<int> rest;
-  for (int d : axes)
-    if (d != f) rest.push_back(d);
-  std::sort(rest.begin(), rest.end(), [&](int a, int b) { return O.h.m[a] < O.h.m[b]; });
-  order.insert(order.end(), rest.begin(), rest.end());
-  const std::vector<int> forder = order;  // the forward pass order
-  if (inverse) std::reverse(order.begin(), order.end());
-  int n = (int)order.size();
-  // Intermediate layouts.  Dense: after the pass over axis order[i] the data
-  // are stored with the next pass's axis fastest and this pass's axis next,
-  // so every pass reads whole contiguous lines and writes rows of T
-  // contiguous elements at the stride of one line; the intermediates
-  // alternate between the workspace halves.  Otherwise (or when a dense
-  // block does not fit a slot) they stay in the output layout, in place.
   static int dense_env = -1;
   if (dense_env < 0) {
     const char *e = getenv("DM_DENSE_INTERMEDIATES");
     dense_env = e ? atoi(e) : 1;
   }
-  // mids[k]: the layout after forward pass k.  The backward program runs the
-  // forward's passes in reverse, through the same layouts.
-  std::vector<dm::Side> mids;
+  const int n = (int)axes.size();
   bool dense = dense_env != 0 && n >= 2;
+  // pass order: dense -- the first-read side's fastest axis, then the rest
+  // by the last-written side's strides; otherwise the forward order
+  // (start's fastest axis first, the rest by output strides), reversed for
+  // the backward
+  auto make_order = [&](const dm::Side &first, const dm::Side &last) {
+    const int f = first.h.fastest;
+    std::vector<int> ord;
+    if (std::find(axes.begin(), axes.end(), f) != axes.end()) ord.push_back(f);
+    std::vector<int> rest;
+    for (int d : axes)
+      if (d != f) rest.push_back(d);
+    std::sort(rest.begin(), rest.end(), [&](int a, int b) { return last.h.m[a] < last.h.m[b]; });
+    ord.insert(ord.end(), rest.begin(), rest.end());
+    return ord;
+  };
+  auto fallback_order = [&]() {
+    std::vector<int> ord = make_order(S, O);
+    if (inverse) std::reverse(ord.begin(), ord.end());
+    return ord;
+  };
+  const dm::Side &last = inverse ? S : O;
+  std::vector<int> order = dense ? (inverse ? make_order(O, S) : make_order(S, O)) : fallback_order();
+  // Dense intermediates: after the pass over axis order[i] the data are
+  // stored with the next pass's axis fastest and this pass's axis next, so
+  // the next pass reads whole lines; they alternate between the workspace
+  // halves, ending in W0 (forward) or leaving W0 first (backward).
+  std::vector<dm::Side> mids;
   if (dense) {
     const i64 region = (i64)plan->extent;
     for (int i = 0; i + 1 < n && dense; i++) {
-      int cur = forder[i], next = forder[i + 1];
+      int cur = order[i], next = order[i + 1];
       std::vector<int> ord = {next, cur};
-      std::vector<int> rest;
+      std::vector<int> others;
       for (int d = 0; d < nd; d++)
-        if (d != cur && d != next) rest.push_back(d);
-      std::sort(rest.begin(), rest.end(), [&](int a, int b) { return O.h.m[a] < O.h.m[b]; });
-      ord.insert(ord.end(), rest.begin(), rest.end());
+        if (d != cur && d != next) others.push_back(d);
+      std::sort(others.begin(), others.end(), [&](int a, int b) { return last.h.m[a] < last.h.m[b]; });
+      ord.insert(ord.end(), others.begin(), others.end());
       dm::HostLayout h;
       if (!dense_layout(x->L, ord, plan->comm->world->nranks, region, &h)) dense = false;
       else mids.push_back(dm::make_side(h, O.slot));
+    }
+    if (!dense) {  // a block did not fit: the in-place scheme
+      mids.clear();
+      order = fallback_order();
     }
   }
   dm::TableBuilder tb;
@@ -321,21 +338,21 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
     void *bin, *bout;
     const dm::Side *si, *so;
     if (dense) {
-      // intermediate i (after pass i) in W1 when n - 2 - i is even, else W0,
-      // so that the last pass writes W0 (forward) from the other half
-      auto mid_buf = [&](int k) { return ((n - 2 - k) % 2 == 0) ? w1 : w0; };
       if (!inverse) {
+        // intermediate k in W1 when n - 2 - k is even, else W0, so that the
+        // last pass writes W0 from the other half
+        auto mid_buf = [&](int k) { return ((n - 2 - k) % 2 == 0) ? w1 : w0; };
         bin = (i == 0) ? dat : prev;
         si = (i == 0) ? &S : &mids[i - 1];
         bout = (i == n - 1) ? w0 : mid_buf(i);
         so = (i == n - 1) ? &O : &mids[i];
       } else {
-        // backward pass i undoes forward pass n-1-i: from the layout after
-        // it to the layout before it
+        // intermediate k in W1 when k is even, else W0: the first pass
+        // leaves W0, which holds its input
         bin = (i == 0) ? w0 : prev;
-        si = (i == 0) ? &O : &mids[n - 1 - i];
+        si = (i == 0) ? &O : &mids[i - 1];
         bout = (i == n - 1) ? dat : ((i % 2 == 0) ? w1 : w0);
-        so = (i == n - 1) ? &S : &mids[n - 2 - i];
+        so = (i == n - 1) ? &S : &mids[i];
       }
     } else if (!inverse) {
       bout = w0;
@@ -623,11 +640,17 @@ static double model_cost(SplitPlan p) {
   for (int g = 0; g < ho.L[f]; g++) nloc += ho.rank_part[f][g] == 0;
   double run = std::max(1, nloc) * es;
   if (run < 128.0) cost *= 128.0 / run;
+  // the backward's first pass reads whole lines of that axis: a line split
+  // over ranks reads as many short pieces from as many slots (measured at
+  // 512^3: 1.0 ms against 0.73 ms for a local line)
+  if (ho.L[f] > 1) cost *= 1.0 + 0.3 * (1.0 - (double)nloc / ho.L[f]);
   // fewer, larger rank blocks keep the workspace compact
   cost *= 1.0 + 1e-3 * p->extent / std::max(1.0, N);
   return cost;
 }
 
+// Milliseconds of a forward + backward step of plan p (the backward
+// overwrites dat: the caller saves it).
 static double time_plan(SplitPlan p, int reps) {
   if (!compile_plan(p)) return 1e30;
   void *s = dm::world_stream(p->comm->world);
@@ -636,8 +659,9 @@ static double time_plan(SplitPlan p, int reps) {
   (void)hipEventCreate(&e1);
   bool ok = true;
   auto run = [&]() {
-    for (auto &L : p->ext->fwd.passes)
-      if (ok) ok = dm::launch_ok(p->comm, dm::launch_fft_pass(L, s), "FFT pass launch");
+    for (auto *prog : {&p->ext->fwd, &p->ext->bwd})
+      for (auto &L : prog->passes)
+        if (ok) ok = dm::launch_ok(p->comm, dm::launch_fft_pass(L, s), "FFT pass launch");
   };
   run();
   (void)hipEventRecord(e0, (hipStream_t)s);
@@ -701,17 +725,33 @@ static SplitPlan autotune(int dbl, Decomp start, void *dat, int length, void *wo
   quiet.reset();
   if (cands.empty()) return nullptr;
   std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.cost < b.cost; });
-  size_t keep = std::min<size_t>(cands.size(), 4);
+  size_t keep = std::min<size_t>(cands.size(), 6);
   SplitPlan best = cands[0].plan;
   if (dm::device_available() && keep > 1) {
-    double bt = 1e30;
-    for (size_t i = 0; i < keep; i++) {
-      double t = time_plan(cands[i].plan, 3);
-      if (t < bt) {
-        bt = t;
-        best = cands[i].plan;
-      }
+    // The candidates' backward passes write dat: keep a copy of the
+    // caller's data (every rank slot) and put it back afterwards.  Without
+    // room for the copy only the model decides.
+    dm::Buffer bd;
+    size_t od;
+    void *save = nullptr;
+    size_t bytes = 0;
+    if (dm::resolve(dat, &bd, &od)) {
+      bytes = bd.slot * (size_t)bd.nranks;
+      save = dm::device_alloc(bytes);
     }
+    if (save != nullptr && hipMemcpy(save, bd.base, bytes, hipMemcpyDeviceToDevice) == hipSuccess) {
+      double bt = 1e30;
+      for (size_t i = 0; i < keep; i++) {
+        double t = time_plan(cands[i].plan, 2);
+        if (t < bt) {
+          bt = t;
+          best = cands[i].plan;
+        }
+      }
+      if (hipMemcpy(bd.base, save, bytes, hipMemcpyDeviceToDevice) != hipSuccess)
+        raise_error(start->comm, "autotune: restoring the input after the timing runs failed");
+    }
+    dm::device_free(save);
   }
   for (auto &c : cands)
     if (c.plan != best) freeSplitPlan(c.plan);
