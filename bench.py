@@ -459,6 +459,7 @@ def main():
     ap.add_argument("--no-check", action="store_true",
                     help="skip the FP64 reference spectrum and the error checks (profiling runs)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-phases", action="store_true", help="skip the phase-order timing")
     ap.add_argument("--no-reshape", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -532,6 +533,28 @@ def main():
     dat.free()
     work.free()
 
+    # the reference's default split (A = 16, B = 32 on every axis) in the
+    # reference's phase order, each phase timed, beside the fused program
+    # for the same split
+    phases = None
+    if not args.no_phases:
+        print("bench: phase order, default split", file=sys.stderr, flush=True)
+        _, datp, workp, planp, _ = setup(C, dm, comm, True, host_x, [32, 32, 32])
+        wall_f, per_f, _ = time_steps(C, dm, comm, planp, 5, 2)
+        C.dm_split_set_phase_order(planp, 1)
+        chkp = verify(planp, datp, workp, per, True) if X is not None else None
+        checks_ok = checks_ok and (chkp is None or chkp["ok"])
+        wall_p, per_p, timers_p = time_steps(C, dm, comm, planp, 5, 2)
+        phases = {"workload": "the same 512^3 transform with the reference's default split (A = 16, B = 32 per "
+                              "axis), forward+inverse",
+                  "phase_order_ms_per_step": round(wall_p / 5 * 1e3, 4),
+                  "timers_ms_per_step": timers_p,
+                  "fused_ms_per_step_same_split": round(wall_f / 5 * 1e3, 4),
+                  "checks": chkp}
+        C.freeSplitPlan(planp)
+        datp.free()
+        workp.free()
+
     fp32 = None
     if not args.no_fp32:
         print("bench: planning + timing FP32", file=sys.stderr, flush=True)
@@ -599,7 +622,7 @@ def main():
                                "{Local(512), Block(512,4) rs8, Block(512,8) rs1} on 32 virtual ranks",
                    "A_factors": A64, "gflops": round(2 * 5 * N * np.log2(N) / (wall / args.steps) / 1e9, 1),
                    "step_ms_events": step_stats(per_step), "timers_ms_per_step": timers64,
-                   "checks": chk64, "fp32_form": fp32, "reshape_chain": chain,
+                   "checks": chk64, "phase_order": phases, "fp32_form": fp32, "reshape_chain": chain,
                    "batched": batched, "serving": serving},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),

@@ -351,3 +351,93 @@ bool layout_usable(dm_comm_t comm, const char *what, const HostLayout &h, size_t
 }
 
 }  // namespace dm
+
+namespace dm {
+
+// Axis d of h as two sub-axes over g = y + A_d * z: sub-axis 2d is y (A_d
+// points), sub-axis 2d+1 is z (L_d / A_d points).  The owner and the local
+// offset must be sums of a y part and a z part (the split FFT's A and B
+// decompositions are: Interleave(Block(A, ga), B) gives rank(y) and
+// z + B * off(y); BlockWithFactor(A * B, A, gb) gives rank(z) and
+// y + A * (z % (bs / A))).  Each part's offsets are divided by their common
+// step, which goes into the sub-axis' memory stride.
+bool split_layout(const HostLayout &h, const std::vector<int> &A, HostLayout *out, std::string *why) {
+  const int nd = h.ndim;
+  if (2 * nd > DM_MAX_DIMS) {
+    *why = "too many axes to split";
+    return false;
+  }
+  HostLayout s;
+  s.ndim = 2 * nd;
+  s.proc_used = h.proc_used;
+  s.L.assign(2 * nd, 1);
+  s.rank_part.assign(2 * nd, std::vector<int>());
+  s.off.assign(2 * nd, std::vector<int>());
+  s.strides.assign((size_t)h.proc_used * 2 * nd, 0);
+  s.uniform = h.uniform;
+  s.slot_extent = h.slot_extent;
+  s.offsets_ok = h.offsets_ok;
+  for (int d = 0; d < nd; d++) {
+    const int L = h.L[d], a = A[d];
+    if (a < 1 || L % a != 0) {
+      *why = "factor does not divide the axis";
+      return false;
+    }
+    const int b = L / a;
+    const auto &rp = h.rank_part[d];
+    const auto &of = h.off[d];
+    std::vector<int> ry(a), rz(b), oy(a), oz(b);
+    for (int z = 0; z < b; z++) {
+      rz[z] = rp[a * z];
+      oz[z] = of[a * z];
+    }
+    for (int y = 0; y < a; y++) {
+      ry[y] = rp[y] - rp[0];
+      oy[y] = of[y] - of[0];
+    }
+    for (int z = 0; z < b; z++)
+      for (int y = 0; y < a; y++) {
+        const int g = y + a * z;
+        if (rp[g] >= DM_NO_OWNER || rp[g] != ry[y] + rz[z] || of[g] != oy[y] + oz[z]) {
+          *why = "the decomposition is not separable along the split";
+          return false;
+        }
+      }
+    auto step_of = [](const std::vector<int> &o) {
+      int st = 0;
+      for (int v : o) st = std::gcd(st, v < 0 ? -v : v);
+      return st > 0 ? st : 1;
+    };
+    const int sy = step_of(oy), sz = step_of(oz);
+    for (auto &v : oy) v /= sy;
+    for (auto &v : oz) v /= sz;
+    s.L[2 * d] = a;
+    s.L[2 * d + 1] = b;
+    s.rank_part[2 * d] = ry;
+    s.rank_part[2 * d + 1] = rz;
+    s.off[2 * d] = oy;
+    s.off[2 * d + 1] = oz;
+    for (int r = 0; r < h.proc_used; r++) {
+      const i64 m = h.strides[(size_t)r * nd + d];
+      s.strides[(size_t)r * 2 * nd + 2 * d] = m * sy;
+      s.strides[(size_t)r * 2 * nd + 2 * d + 1] = m * sz;
+    }
+  }
+  s.m.assign(s.strides.begin(), s.strides.begin() + 2 * nd);
+  // fastest sub-axis: smallest stride among those longer than one; ties ->
+  // the later sub-axis
+  int best = 2 * nd - 1;
+  i64 bs = -1;
+  for (int i = 2 * nd - 1; i >= 0; i--) {
+    if (s.L[i] <= 1) continue;
+    if (bs < 0 || s.m[i] < bs) {
+      bs = s.m[i];
+      best = i;
+    }
+  }
+  s.fastest = best;
+  *out = s;
+  return true;
+}
+
+}  // namespace dm

@@ -55,6 +55,11 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
 void *finalize_passes(dm_comm_t comm, TableBuilder &tb, std::vector<PendingPass> &pp,
                       std::vector<FftLaunch> *out);
 
+// Axis d of h split into sub-axes 2d (y, A[d] points) and 2d+1 (z, L_d /
+// A[d] points) over g = y + A[d] * z, when owner and offset separate into
+// y and z parts; false (with the reason) otherwise.
+bool split_layout(const HostLayout &h, const std::vector<int> &A, HostLayout *out, std::string *why);
+
 // Layout validity for FFT use: uniform strides, every coordinate owned,
 // every rank's data inside region_elems.
 bool layout_usable(dm_comm_t comm, const char *what, const HostLayout &h, size_t region_elems);

@@ -98,6 +98,24 @@ int launch_fft_pass(const FftLaunch &p, void *stream);
 // 512 points, 256-byte rows).
 int fft_tile_capacity(int dbl, int column, int wide = 0);
 
+// ---- K4: twiddle multiply over every rank's local block -------------------
+// Rank r's block: count positions per axis at its memory strides; the
+// factor of axis d at local position p is tw[d][first + p] (complex double,
+// (re, im) pairs), multiplied outer axis to inner; zero products skip the
+// slot; conj multiplies by the conjugate.
+struct ScaleLaunch {
+  void *data;                  // slot 0 of the buffer (pass precision)
+  i64 slot;                    // elements per rank slot
+  int ndim;
+  int nranks;
+  const int *rank_tab;         // [nranks][ndim][2]: first table row, count
+  const i64 *strides;          // [nranks][ndim]
+  const double *tw[DM_MAX_DIMS];
+  long long max_elems;         // largest per-rank block
+  int dbl, conj;
+};
+int launch_scale(const ScaleLaunch &p, void *stream);
+
 // ---- K3: batched local FFT pass (many transforms, one axis of each) --------
 // A tile holds nl (<= T) lines of N points of one transform: point x of line
 // l at element base + l * (row ? N : 1) + x * ps of that transform's array.

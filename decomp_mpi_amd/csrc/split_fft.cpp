@@ -190,20 +190,62 @@ struct dm_fft_program {
   void *mem = nullptr;  // device tables
 };
 
+// The reference's phase order on the device: reshape (K1), stage DFTs over
+// sub-axes (K2), twiddle multiply (K4), reshape, stage DFTs.
+struct dm_phased_program {
+  bool compiled = false;
+  dm_fft_program st1_fwd, st1_bwd, st2_fwd, st2_bwd;  // stage passes, in place
+  void *tw_mem = nullptr;   // twiddle tables and the per-rank table of K4
+  bool has_twiddle = false;
+  dm::ScaleLaunch scale{};
+  ReshapePlan a_to_b = nullptr, b_to_a = nullptr;  // owned: plans whose A_to_B the reference leaves NULL
+};
+
 struct dm_split_ext {
   int dbl = 1;
   int ndim = 0;
   std::vector<int> A, B, L;
   Decomp start = nullptr;   // caller's (aliased)
   Decomp out = nullptr;     // B_flipped, or A for single-stage plans (owned by the plan)
+  bool use_first = false;   // single stage: output in the A decomposition
   bool compiled = false;
   dm_fft_program fwd, bwd;
+  bool phased = false;      // run in the reference's phase order
+  bool fused_ok = true;     // every axis fits one pass (else phased only)
+  dm_phased_program ph;
 };
 
 static void program_release(dm_fft_program &p) {
   if (p.mem) dm::device_free(p.mem);
   p.mem = nullptr;
   p.passes.clear();
+}
+
+static void phased_release(dm_phased_program &ph) {
+  program_release(ph.st1_fwd);
+  program_release(ph.st1_bwd);
+  program_release(ph.st2_fwd);
+  program_release(ph.st2_bwd);
+  if (ph.tw_mem) dm::device_free(ph.tw_mem);
+  ph.tw_mem = nullptr;
+  freeReshapePlan(ph.a_to_b);
+  freeReshapePlan(ph.b_to_a);
+  ph.a_to_b = ph.b_to_a = nullptr;
+  ph.compiled = false;
+}
+
+// split_fft.c:373-393: w = exp(-2 pi i y l / L) with the quarter turns exact.
+static void twiddle_value(int y, int l, int L, double *re, double *im) {
+  long long rem = ((long long)y * l) % L;
+  if (rem == 0) *re = 1.0, *im = 0.0;
+  else if (4 * rem == L) *re = 0.0, *im = -1.0;
+  else if (2 * rem == L) *re = -1.0, *im = 0.0;
+  else if (4 * rem == 3LL * L) *re = -0.0, *im = 1.0;
+  else {
+    double angle = -(2.0 * M_PI) * ((double)((long long)y * l) / (double)L);
+    *re = cos(angle);
+    *im = sin(angle);
+  }
 }
 
 // A dense intermediate layout: axes in `order` (fastest first), the slowest
@@ -375,35 +417,52 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
   return true;
 }
 
-static bool compile_plan(SplitPlan plan) {
+// The plan's dat and workspace as registered buffers, checked against the
+// decompositions they hold.
+static bool bind_buffers(SplitPlan plan, dm::Buffer *bd, size_t *od, dm::Buffer *bw, size_t *ow) {
   dm_split_ext *x = plan->ext;
-  if (x->compiled) return true;
   dm_comm_t comm = plan->comm;
   size_t es = x->dbl ? 16 : 8;
-  dm::Buffer bd, bw;
-  size_t od, ow;
-  if (!dm::resolve(plan->src, &bd, &od) || !dm::resolve(plan->work, &bw, &ow)) {
+  if (!dm::resolve(plan->src, bd, od) || !dm::resolve(plan->work, bw, ow)) {
     raise_error(comm, "split FFT: dat and workspace must point into distributed device buffers "
                       "(dm_malloc/dm_register)");
     return false;
   }
-  if (od % es || ow % es || bd.slot % es || bw.slot % es) {
+  if (*od % es || *ow % es || bd->slot % es || bw->slot % es) {
     raise_error(comm, "split FFT: buffers must be aligned to the element size");
     return false;
   }
   dm::HostLayout hs = dm::layout_from_decomp(x->start);
   dm::HostLayout ho = dm::layout_from_decomp(x->out);
-  if (bd.nranks < hs.proc_used || bw.nranks < ho.proc_used) {
+  if (bd->nranks < hs.proc_used || bw->nranks < ho.proc_used) {
     raise_error(comm, "split FFT: buffers have fewer rank slots than the decompositions use");
     return false;
   }
   size_t ext = (size_t)plan->extent;
-  if (ow + 2 * ext * es > bw.slot) {
-    raise_error(comm, "split FFT: workspace slot of %zu bytes cannot hold 2 x %zu elements", bw.slot - ow, ext);
+  if (*ow + 2 * ext * es > bw->slot) {
+    raise_error(comm, "split FFT: workspace slot of %zu bytes cannot hold 2 x %zu elements", bw->slot - *ow, ext);
     return false;
   }
-  if (!dm::layout_usable(comm, "start", hs, (bd.slot - od) / es) || !dm::layout_usable(comm, "output", ho, ext))
+  if (!dm::layout_usable(comm, "start", hs, (bd->slot - *od) / es) || !dm::layout_usable(comm, "output", ho, ext))
     return false;
+  return true;
+}
+
+static bool compile_plan(SplitPlan plan) {
+  dm_split_ext *x = plan->ext;
+  if (x->compiled) return true;
+  if (!x->fused_ok) {
+    raise_error(plan->comm, "split FFT: an axis is longer than one pass tile; this plan runs in phase order only");
+    return false;
+  }
+  dm_comm_t comm = plan->comm;
+  size_t es = x->dbl ? 16 : 8;
+  dm::Buffer bd, bw;
+  size_t od, ow;
+  if (!bind_buffers(plan, &bd, &od, &bw, &ow)) return false;
+  size_t ext = (size_t)plan->extent;
+  dm::HostLayout hs = dm::layout_from_decomp(x->start);
+  dm::HostLayout ho = dm::layout_from_decomp(x->out);
   dm::Side S = dm::make_side(hs, (i64)(bd.slot / es));
   dm::Side O = dm::make_side(ho, (i64)(bw.slot / es));
   char *dat = bd.base + od;
@@ -415,6 +474,217 @@ static bool compile_plan(SplitPlan plan) {
   plan->plan_b_back = &x->bwd;
   x->compiled = true;
   return true;
+}
+
+// ---------------------------------------------------------------- phase order
+// The reference's runSplitPlan phases (split_fft.c:622-682), each its own
+// device work: RESHAPE_1 start -> A (K1), FFT_1 the B_d-point DFTs along
+// the z sub-axes of the A layout (K2, in place), TWIDDLE the multiply by
+// the plan's twiddle tables (K4), RESHAPE_2 A -> B (K1), FFT_2 the
+// A_d-point DFTs along the y sub-axes of the B layout (K2, in place).  The
+// result lands in work[0:extent) in the B layout, as the fused program's.
+// The backward runs the exact inverse in reverse order (FFT_2, RESHAPE_2
+// B -> A, conjugate TWIDDLE, FFT_1, RESHAPE_1 A -> start).  Each phase is
+// timed into plan->times under its own name.
+
+// Stage passes over the sub-axes `subs` of a split layout, in place on buf,
+// fastest sub-axis first.
+static bool compile_stage(SplitPlan plan, const dm::Side &side, std::vector<int> subs, bool inverse, void *buf,
+                          dm_fft_program *prog) {
+  prog->passes.clear();
+  prog->mem = nullptr;
+  prog->inverse = inverse ? 1 : 0;
+  if (subs.empty()) return true;
+  std::stable_sort(subs.begin(), subs.end(), [&](int a, int b) { return side.h.m[a] < side.h.m[b]; });
+  dm::TableBuilder tb;
+  std::vector<dm::PendingPass> pp(subs.size());
+  for (size_t i = 0; i < subs.size(); i++)
+    if (!dm::plan_pass(plan->comm, side, side, subs[i], plan->ext->dbl, inverse, buf, buf, tb, &pp[i])) return false;
+  prog->mem = dm::finalize_passes(plan->comm, tb, pp, &prog->passes);
+  return prog->mem != nullptr;
+}
+
+static bool compile_phased(SplitPlan plan) {
+  dm_split_ext *x = plan->ext;
+  dm_phased_program &ph = x->ph;
+  if (ph.compiled) return true;
+  dm_comm_t comm = plan->comm;
+  const int nd = x->ndim;
+  const size_t es = x->dbl ? 16 : 8;
+  dm::Buffer bd, bw;
+  size_t od, ow;
+  if (!bind_buffers(plan, &bd, &od, &bw, &ow)) return false;
+  char *w0 = bw.base + ow;
+  char *w1 = w0 + (size_t)plan->extent * es;
+  const bool two = !x->use_first;
+  std::string why;
+  dm::HostLayout hA = dm::layout_from_decomp(plan->A), sA;
+  if (!dm::layout_usable(comm, "A", hA, (size_t)plan->extent)) return false;
+  if (!dm::split_layout(hA, x->A, &sA, &why)) {
+    raise_error(comm, "split FFT: A decomposition: %s", why.c_str());
+    return false;
+  }
+  dm::Side SA = dm::make_side(sA, (i64)(bw.slot / es));
+  std::vector<int> z_subs, y_subs;
+  for (int d = 0; d < nd; d++) {
+    if (x->B[d] > 1) z_subs.push_back(2 * d + 1);
+    if (x->A[d] > 1) y_subs.push_back(2 * d);
+  }
+  void *st1_buf = two ? w1 : w0;
+  if (!compile_stage(plan, SA, z_subs, false, st1_buf, &ph.st1_fwd) ||
+      !compile_stage(plan, SA, z_subs, true, st1_buf, &ph.st1_bwd))
+    return false;
+  if (two) {
+    dm::HostLayout hB = dm::layout_from_decomp(plan->B), sB;
+    if (!dm::layout_usable(comm, "B", hB, (size_t)plan->extent)) return false;
+    if (!dm::split_layout(hB, x->A, &sB, &why)) {
+      raise_error(comm, "split FFT: B decomposition: %s", why.c_str());
+      return false;
+    }
+    dm::Side SB = dm::make_side(sB, (i64)(bw.slot / es));
+    if (!compile_stage(plan, SB, y_subs, false, w0, &ph.st2_fwd) ||
+        !compile_stage(plan, SB, y_subs, true, w0, &ph.st2_bwd))
+      return false;
+    if (plan->A_to_B == nullptr) {  // single second stage: the reference has no A -> B plan
+      MPI_Datatype base = x->dbl ? &dm_type_c_double_complex : &dm_type_c_float_complex;
+      ph.a_to_b = makeReshapePlan(comm, base, plan->A, plan->B);
+      ph.b_to_a = makeReshapePlan(comm, base, plan->B, plan->A);
+      if (ph.a_to_b == nullptr || ph.b_to_a == nullptr) return false;
+    }
+  }
+  // twiddle tables of every rank position, rows in position order, and the
+  // per-rank (first row, count) table with the A layout's strides
+  ph.has_twiddle = two && plan->twiddle != nullptr;
+  if (ph.has_twiddle) {
+    std::vector<std::vector<double>> tabs(nd);
+    std::vector<std::vector<int>> first(nd);
+    for (int d = 0; d < nd; d++) {
+      AxisDecomp ax = plan->A->dims[d];
+      const int ps = ax->decomp->proc_size;
+      first[d].assign(ps + 1, 0);
+      for (int p = 0; p < ps; p++) first[d][p + 1] = first[d][p] + ax->ext->max_local_pos[p] + 1;
+      tabs[d].assign(2 * (size_t)first[d][ps], 0.0);
+      for (int g = 0; g < x->L[d]; g++) {
+        const int p = ax->ext->rank_of[g], local = ax->ext->off_of[g];
+        if (p < 0 || p >= ps || local < 0 || local > ax->ext->max_local_pos[p]) continue;
+        double re, im;
+        twiddle_value(g % x->A[d], g / x->A[d], x->L[d], &re, &im);
+        tabs[d][2 * (size_t)(first[d][p] + local)] = re;
+        tabs[d][2 * (size_t)(first[d][p] + local) + 1] = im;
+      }
+    }
+    const int nr = plan->A->proc_used;
+    std::vector<int> rt((size_t)nr * nd * 2);
+    long long max_elems = 0;
+    for (int r = 0; r < nr; r++) {
+      long long n = 1;
+      for (int d = 0; d < nd; d++) {
+        AxisDecomp ax = plan->A->dims[d];
+        const int pos = (r / ax->rank_stride) % ax->decomp->proc_size;
+        rt[((size_t)r * nd + d) * 2] = first[d][pos];
+        rt[((size_t)r * nd + d) * 2 + 1] = ax->ext->max_local_pos[pos] + 1;
+        n *= ax->ext->max_local_pos[pos] + 1;
+      }
+      max_elems = std::max(max_elems, n);
+    }
+    dm::TableBuilder tb;
+    std::vector<size_t> o_tab(nd);
+    for (int d = 0; d < nd; d++) o_tab[d] = tb.put(tabs[d].data(), tabs[d].size() * sizeof(double));
+    const size_t o_rt = tb.put(rt.data(), rt.size() * sizeof(int));
+    const size_t o_st = tb.put(hA.strides.data(), hA.strides.size() * sizeof(i64));
+    ph.tw_mem = dm::device_alloc(tb.host.size());
+    if (ph.tw_mem == nullptr) {
+      raise_error(comm, "split FFT: device allocation of %zu bytes of twiddle tables failed", tb.host.size());
+      return false;
+    }
+    dm::h2d(ph.tw_mem, tb.host.data(), tb.host.size());
+    char *b = (char *)ph.tw_mem;
+    dm::ScaleLaunch &L = ph.scale;
+    L = dm::ScaleLaunch{};
+    L.data = w1;
+    L.slot = (i64)(bw.slot / es);
+    L.ndim = nd;
+    L.nranks = nr;
+    L.rank_tab = (const int *)(b + o_rt);
+    L.strides = (const i64 *)(b + o_st);
+    for (int d = 0; d < nd; d++) L.tw[d] = (const double *)(b + o_tab[d]);
+    L.max_elems = max_elems;
+    L.dbl = x->dbl;
+  }
+  ph.compiled = true;
+  return true;
+}
+
+// Run one direction in phase order; times[] gets each phase's HIP-event
+// time when the world is synchronous.
+static void run_phased(SplitPlan plan, int direction) {
+  dm_split_ext *x = plan->ext;
+  dm_phased_program &ph = x->ph;
+  dm_world *w = plan->comm->world;
+  void *s = dm::world_stream(w);
+  const bool timed = !w->async;
+  const size_t es = x->dbl ? 16 : 8;
+  const bool two = !x->use_first;
+  char *w0 = (char *)plan->work;
+  char *w1 = w0 + (size_t)plan->extent * es;
+  ReshapePlan a_to_b = plan->A_to_B ? plan->A_to_B : ph.a_to_b;
+  ReshapePlan b_to_a = plan->B_to_A ? plan->B_to_A : ph.b_to_a;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+  }
+  bool ok = true;
+  double total = 0;
+  auto phase = [&](int timer, const std::function<bool()> &work) {
+    if (!ok) return;
+    if (timed) (void)hipEventRecord(e0, (hipStream_t)s);
+    ok = work();
+    if (!timed || !ok) return;
+    (void)hipEventRecord(e1, (hipStream_t)s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    plan->times[timer] += ms * 1e-3;
+    total += ms * 1e-3;
+  };
+  auto passes = [&](dm_fft_program &prog) {
+    return [&]() {
+      for (auto &L : prog.passes)
+        if (!dm::launch_ok(plan->comm, dm::launch_fft_pass(L, s), "FFT pass launch")) return false;
+      return true;
+    };
+  };
+  auto scale = [&](int conj) {
+    return [&, conj]() {
+      dm::ScaleLaunch L = ph.scale;
+      L.conj = conj;
+      return dm::launch_ok(plan->comm, dm::launch_scale(L, s), "twiddle kernel launch");
+    };
+  };
+  void *st1_buf = two ? w1 : w0;
+  if (direction == FFTW_FORWARD) {
+    phase(RESHAPE_1, [&]() { return dm::enqueue_reshape(plan->comm, plan->start_to_A, plan->src, st1_buf, s); });
+    phase(FFT_1, passes(ph.st1_fwd));
+    if (ph.has_twiddle) phase(TWIDDLE, scale(0));
+    if (two) {
+      phase(RESHAPE_2, [&]() { return dm::enqueue_reshape(plan->comm, a_to_b, w1, w0, s); });
+      phase(FFT_2, passes(ph.st2_fwd));
+    }
+  } else {
+    if (two) {
+      phase(FFT_2, passes(ph.st2_bwd));
+      phase(RESHAPE_2, [&]() { return dm::enqueue_reshape(plan->comm, b_to_a, w0, w1, s); });
+    }
+    if (ph.has_twiddle) phase(TWIDDLE, scale(1));
+    phase(FFT_1, passes(ph.st1_bwd));
+    phase(RESHAPE_1, [&]() { return dm::enqueue_reshape(plan->comm, plan->A_to_start, st1_buf, plan->src, s); });
+  }
+  if (timed) {
+    plan->times[ALL] += total;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
 }
 
 // ---------------------------------------------------------------- planner
@@ -436,14 +706,20 @@ static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, D
       raise_error(comm, "Invalid factor");
       return nullptr;
     }
-  // The device passes transform whole axes: an axis they cannot do rejects
-  // the plan here, as the reference rejects plans it cannot build
+  // The fused passes transform whole axes; the phase order transforms the
+  // A_d- and B_d-point factors of each axis.  An axis longer than one pass
+  // tile makes a phase-order plan; an axis neither form can do rejects the
+  // plan here, as the reference rejects plans it cannot build
   // (split_fft.c:252-254,304-308): a logged reason and NULL.
+  bool fused_ok = true;
   for (int i = 0; i < ndim; i++) {
-    std::string why;
-    if (L[i] > 1 && !dm::axis_supported(L[i], dbl, &why)) {
+    std::string why, why_a, why_b;
+    if (L[i] <= 1 || dm::axis_supported(L[i], dbl, &why)) continue;
+    fused_ok = false;
+    if (!dm::axis_supported(A[i], dbl, &why_a) || !dm::axis_supported(B[i], dbl, &why_b)) {
       logger(comm, "buildParallelPlan: axis %d: %s\n", i, why.c_str());
-      t_rejection = "axis " + std::to_string(i) + ": " + why;
+      t_rejection = "axis " + std::to_string(i) + ": " + why + ", and its factors " + std::to_string(A[i]) + " x " +
+                    std::to_string(B[i]) + " are not both within one pass";
       return nullptr;
     }
   }
@@ -459,6 +735,9 @@ static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, D
   x->B.assign(B, B + ndim);
   x->L.assign(L, L + ndim);
   x->start = start;
+  x->use_first = use_first != 0;
+  x->fused_ok = fused_ok;
+  x->phased = !fused_ok || (flags & RUN_PHASE_ORDER) != 0;
 
   // first decomposition keeps the long-distance factor local (split_fft.c:281)
   plan->A = makeInterleavedDecomp(ndim, comm, A, B, grid_a);
@@ -512,18 +791,8 @@ static SplitPlan build_plan(int dbl, int flags, int use_first, int use_second, D
       for (int g = 0; g < L[i]; g++) {
         if (!isLocal(ax, g)) continue;
         int local = ax->decomp->type_offset(ax->decomp, g);
-        int y = g % A[i], l = g / A[i];
-        long long rem = ((long long)y * l) % L[i];
         double re, im;
-        if (rem == 0) re = 1.0, im = 0.0;
-        else if (4 * rem == L[i]) re = 0.0, im = -1.0;
-        else if (2 * rem == L[i]) re = -1.0, im = 0.0;
-        else if (4 * rem == 3LL * L[i]) re = -0.0, im = 1.0;
-        else {
-          double angle = -(2.0 * M_PI) * ((double)((long long)y * l) / (double)L[i]);
-          re = cos(angle);
-          im = sin(angle);
-        }
+        twiddle_value(g % A[i], g / A[i], L[i], &re, &im);
         if (re != 1.0 && im != 0.0) non_trivial = 1;  // as split_fft.c:396
         __real__ plan->twiddle[i][local] = re;
         __imag__ plan->twiddle[i][local] = im;
@@ -652,13 +921,19 @@ static double model_cost(SplitPlan p) {
 // Milliseconds of a forward + backward step of plan p (the backward
 // overwrites dat: the caller saves it).
 static double time_plan(SplitPlan p, int reps) {
-  if (!compile_plan(p)) return 1e30;
+  if (p->ext->phased ? !compile_phased(p) : !compile_plan(p)) return 1e30;
   void *s = dm::world_stream(p->comm->world);
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   bool ok = true;
   auto run = [&]() {
+    if (p->ext->phased) {  // synchronous phases: the loop below times them whole
+      run_phased(p, FFTW_FORWARD);
+      run_phased(p, FFTW_BACKWARD);
+      ok = !dm::error_pending();
+      return;
+    }
     for (auto *prog : {&p->ext->fwd, &p->ext->bwd})
       for (auto &L : prog->passes)
         if (ok) ok = dm::launch_ok(p->comm, dm::launch_fft_pass(L, s), "FFT pass launch");
@@ -821,9 +1096,28 @@ static void run_split(SplitPlan plan, int direction) {
   }
   dm::check_device(plan->comm);
   if (dm::error_pending() && dm::error_mode() == 1) return;
+  if (plan->ext->phased) {
+    if (compile_phased(plan)) run_phased(plan, direction);
+    return;
+  }
   if (!compile_plan(plan)) return;
   if (direction == FFTW_FORWARD) run_program(plan, plan->ext->fwd, FFT_1, FFT_2);
   else run_program(plan, plan->ext->bwd, FFT_2, FFT_1);
+}
+
+extern "C" int dm_split_set_phase_order(SplitPlan plan, int on) {
+  if (plan == nullptr || plan->ext == nullptr) return DM_ERR_ARG;
+  if (!on && !plan->ext->fused_ok) {
+    raise_error(plan->comm, "dm_split_set_phase_order: an axis is longer than one pass tile; the plan runs in "
+                            "phase order only");
+    return DM_ERR_ARG;
+  }
+  plan->ext->phased = on != 0;
+  return DM_SUCCESS;
+}
+
+extern "C" int dm_split_phase_order(SplitPlan plan) {
+  return (plan != nullptr && plan->ext != nullptr && plan->ext->phased) ? 1 : 0;
 }
 
 extern "C" Decomp getTransformed(SplitPlan plan) { return plan->B_flipped; }
@@ -843,6 +1137,7 @@ extern "C" void freeSplitPlan(SplitPlan plan) {
   if (plan->ext) {
     program_release(plan->ext->fwd);
     program_release(plan->ext->bwd);
+    phased_release(plan->ext->ph);
     delete plan->ext;
   }
   freeDecomp(plan->B_flipped);

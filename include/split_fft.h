@@ -50,12 +50,14 @@ struct split_fft_plan {
 /* timer fields (split_fft.h:32-40) */
 enum Timers { RESHAPE_1 = 0, FFT_1, TWIDDLE, RESHAPE_2, FFT_2, ALL, NUM_TIMERS };
 
-/* plan search flags (split_fft.h:42-47) */
+/* plan search flags (split_fft.h:42-47); RUN_PHASE_ORDER is this library's:
+ * the plan runs in the reference's phase order (see dm_split_set_phase_order) */
 enum SearchFlags {
   NO_FLAGS = 0x00,
   PREFER_LARGE_FFT = 0x01,
   PREFER_STAGE1 = 0x02,
-  FORCE_USE_FIRST = 0x04
+  FORCE_USE_FIRST = 0x04,
+  RUN_PHASE_ORDER = 0x100
 };
 typedef struct split_fft_plan *SplitPlan;
 
@@ -100,6 +102,19 @@ SplitPlan makeSplitPlanf(int slab, Decomp start, fftwf_complex *dat, int length,
                          fftwf_complex *workspace);
 SplitPlan makeParallelPlanf(Decomp start, int *initial_factor, fftwf_complex *dat,
                             int length, fftwf_complex *workspace);
+
+/** No counterpart in the reference.  on = 1: run the plan in the
+ *  reference's phase order, each phase its own device work -- RESHAPE_1
+ *  (start -> A), FFT_1 (the B_d-point DFTs), TWIDDLE (the plan's twiddle
+ *  tables), RESHAPE_2 (A -> B), FFT_2 (the A_d-point DFTs) -- with every
+ *  phase timed into plan->times.  on = 0 (the default): the fused program,
+ *  one pass per axis.  Both give the same result placement.  A plan with an
+ *  axis longer than one pass tile (4096 complex-double or 8192
+ *  complex-float points) whose A_d and B_d each fit one runs in phase order
+ *  only (on = 0 is refused).  Returns 0 or an error code. */
+int dm_split_set_phase_order(SplitPlan plan, int on);
+/** 1 when the plan runs in phase order. */
+int dm_split_phase_order(SplitPlan plan);
 
 /** The decomposition of the forward result: frequency -> rank and offset
  *  in work (split_fft.c:615-621; NULL for single-stage plans). */

@@ -29,17 +29,19 @@ def block_start(shape, grid):
     return [("block", n, g) for n, g in zip(shape, grid)]
 
 
-def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_descs=None, autotune=False):
+def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_descs=None, autotune=False,
+              phased=False, length=None):
     """Forward then backward on the device; forward checked against the
-    oracle on every slot of the output layout and (through getTransformed)
-    against numpy's FP64 DFT of the global array, backward as N * x."""
+    oracle on every slot of the output layout and against numpy's FP64 DFT
+    of the global array, backward as N * x.  phased: the plan runs in the
+    reference's phase order, and every phase's timer must have run."""
     comm = C.dm_world_create(P)
     descs = start_descs or block_start(shape, grid)
     ps, os_ = make_pair(C, comm, P, descs)
     es = 16 if dbl else 8
     per = int(max(os_.extents))
     N = int(np.prod(shape))
-    length = 2 * max(per, N // P + 1) * 4 + 64
+    length = length or 2 * max(per, N // P + 1) * 4 + 64
     dat = dm.DistBuffer(comm, per * es)
     work = dm.DistBuffer(comm, length * es)
     if autotune:
@@ -49,6 +51,8 @@ def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_desc
     else:
         plan = (C.makeParallelPlan if dbl else C.makeParallelPlanf)(ps, dm.ints(B), dat.ptr, length, work.ptr)
     assert plan, "planning failed"
+    if phased:
+        assert C.dm_split_set_phase_order(plan, 1) == 0
     sp = dm.split_plan(plan)
     nd = len(shape)
     A = [sp.A_factors[i] for i in range(nd)]
@@ -61,15 +65,28 @@ def run_split(C, P, shape, grid, B=None, slab=None, dbl=True, seed=0, start_desc
     x = rng.uniform(-1, 1, (P, per)) + 1j * rng.uniform(-1, 1, (P, per))
     dat.put_all(x.astype(np.complex128 if dbl else np.complex64))
     C.runSplitPlan(plan, dm.FFTW_FORWARD)
+    if phased:
+        t = [sp.times[i] for i in range(dm.NUM_TIMERS)]
+        two = any(a > 1 for a in A)
+        for name, i in (("RESHAPE_1", dm.RESHAPE_1), ("FFT_1", dm.FFT_1)):
+            assert t[i] > 0 or (name == "FFT_1" and all(b == 1 for b in Bf)), (name, t)
+        if two:
+            assert t[dm.RESHAPE_2] > 0 and t[dm.FFT_2] > 0, t
+        if two and any(a > 1 and b > 1 for a, b in zip(A, Bf)):
+            assert t[dm.TWIDDLE] > 0, t
+        assert abs(t[dm.ALL] - sum(t[:dm.ALL])) < 1e-9, t
     dt = np.complex128 if dbl else np.complex64
     got = work.get_all(ext * es, dt).astype(np.complex128)
-    # oracle forward on the same input
-    od = x.astype(np.complex128).copy()
-    ow = np.zeros((P, 2 * ext + 16), np.complex128)
-    oracle.split_fft(os_, A, Bf, ga, gb, od, ow, ext, -1)
+    # oracle forward on the same input (the oracle does not restate plans
+    # whose every B_d is 1: those are checked against numpy alone)
     out_d = oracle.split_decomps(P, A, Bf, ga, gb)[1 if any(a > 1 for a in A) else 0]
-    mask = owned_mask(out_d, ext)
-    fwd_err = rel_l2(got[mask], ow[:, :ext][mask])
+    fwd_err = 0.0
+    if not all(b == 1 for b in Bf):
+        od = x.astype(np.complex128).copy()
+        ow = np.zeros((P, 2 * ext + 16), np.complex128)
+        oracle.split_fft(os_, A, Bf, ga, gb, od, ow, ext, -1)
+        mask = owned_mask(out_d, ext)
+        fwd_err = rel_l2(got[mask], ow[:, :ext][mask])
     # independently of the oracle: numpy's DFT of the global array, with
     # frequency k of axis d at output position k / B_d + A_d * (k % B_d)
     # (A = 1 on every axis: the single-stage output in the A layout)
@@ -162,6 +179,43 @@ def test_split_fft_fp32_vs_fp64_oracle(C, P, shape, grid, B):
         fwd, rt, _ = run_split(C, P, shape, grid, B=B, dbl=False)
     assert fwd < TOL32, f"FP32 forward rel L2 {fwd:.3e}"
     assert rt < TOL32
+
+
+PHASED = [
+    (4, (16, 16, 16), (1, 2, 2), [4, 4, 4], None),
+    (4, (16, 16, 16), (1, 2, 2), [4, 2, 8], None),     # A != B
+    (6, (12, 20, 14), (1, 2, 3), [3, 4, 7], None),     # radices 3, 5, 7, uneven blocks
+    (4, (16, 16, 16), (1, 2, 2), [16, 16, 16], None),  # single stage (A = 1)
+    (8, (16, 16, 16), (2, 2, 2), [1, 4, 1], None),     # B_d = 1 beside B_d > 1
+    (2, (8, 8, 8), (1, 1, 2), [1, 1, 1], 4096),        # single second stage (B = 1)
+    (4, (16, 16), (2, 2), [4, 4], None),               # 2D
+]
+
+
+@pytest.mark.parametrize("dbl", [True, False])
+@pytest.mark.parametrize("P,shape,grid,B,length", PHASED)
+def test_split_fft_phase_order_vs_oracle(C, P, shape, grid, B, length, dbl):
+    """The reference's phase order on the device -- reshape, B_d-point
+    DFTs, twiddle multiply with the plan's tables, reshape, A_d-point DFTs,
+    each timed into plan->times -- against the oracle and numpy, and back."""
+    fwd, rt, A = run_split(C, P, shape, grid, B=B, dbl=dbl, phased=True, length=length)
+    tol = TOL64 if dbl else TOL32
+    assert fwd < tol, f"forward rel L2 {fwd:.3e} (A={A})"
+    assert rt < tol, f"round trip {rt:.3e}"
+
+
+@pytest.mark.parametrize("P,shape,grid,B,dbl", [
+    (4, (8192, 8), (2, 2), [128, 8], True),      # 64 x 128 along axis 0: longer than a pass tile
+    (2, (16384, 4), (2, 1), [128, 4], False),    # complex float: 128 x 128
+])
+def test_split_fft_axis_longer_than_a_tile(C, P, shape, grid, B, dbl):
+    """An axis longer than one pass tile plans (in phase order only: the
+    plan's own A x B split is what makes it reachable) and matches the
+    oracle and numpy's DFT."""
+    fwd, rt, A = run_split(C, P, shape, grid, B=B, dbl=dbl)
+    tol = TOL64 if dbl else TOL32
+    assert fwd < tol, f"forward rel L2 {fwd:.3e} (A={A})"
+    assert rt < tol, f"round trip {rt:.3e}"
 
 
 @pytest.mark.parametrize("dbl", [True, False])

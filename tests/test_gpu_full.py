@@ -113,16 +113,18 @@ def c3_data():
     return x, dft3(x), modes
 
 
-PLANS = [("default_B32", [32, 32, 32]), ("B_1_1_256", [1, 1, 256]), ("autotuned", None)]
+PLANS = [("default_B32", [32, 32, 32]), ("B_1_1_256", [1, 1, 256]), ("autotuned", None),
+         ("phase_order_B32", [32, 32, 32])]
 
 
 @pytest.mark.parametrize("dbl", [1, 0])
 @pytest.mark.parametrize("name,factors", PLANS, ids=[p[0] for p in PLANS])
 def test_split_fft_512_vs_fp64_dft(C, c3_data, dbl, name, factors):
     """The reference's default split (A = 16, B = 32 on every axis), the
-    mixed family B = (1, 1, 256) and whatever autotuneParallelPlan[f]
-    returns (the plan bench.py times), each against the FP64 DFT, the
-    planted modes and the round trip on every rank."""
+    mixed family B = (1, 1, 256), whatever autotuneParallelPlan[f] returns
+    (the plan bench.py times) and the default split run in the reference's
+    phase order, each against the FP64 DFT, the planted modes and the
+    round trip on every rank."""
     x, X, modes = c3_data
     comm = C.dm_world_create(P)
     start = C.makePackedDecomp(comm, 3, dm.ptrs([C.makeLocalDecomp(L), C.makeBlockDecomp(L, 4),
@@ -150,6 +152,8 @@ def test_split_fft_512_vs_fp64_dft(C, c3_data, dbl, name, factors):
         make = C.makeParallelPlan if dbl else C.makeParallelPlanf
         plan = make(start, dm.ints(factors), dat.ptr, length, work.ptr)
     assert plan
+    if name.startswith("phase_order"):
+        assert C.dm_split_set_phase_order(plan, 1) == 0
     sp = dm.split_plan(plan)
     A = [sp.A_factors[i] for i in range(3)]
     if factors is not None:

@@ -167,44 +167,85 @@ def test_visit_data_reports_every_slot(C):
     C.dm_world_free(comm)
 
 
-@pytest.mark.parametrize("L,dbl,ok", [(272, 1, False), (17 * 8, 0, False), (8192, 1, False), (8192, 0, True),
-                                      (4096, 1, True), (16 * 27 * 5, 1, True)])
-def test_split_plan_rejects_unsupported_axis_at_plan_time(C, L, dbl, ok):
-    """An axis the device passes cannot transform (a prime factor above 13,
-    or longer than one tile) makes buildParallelPlan return NULL, as the
-    reference's other plan rejections do (split_fft.c:252-308), and
-    makeSplitPlan raise 'Cannot generate plan' with the reason; a supported
-    one plans."""
+@pytest.mark.parametrize("L,dbl,A,ok,phased", [
+    (272, 1, 16, False, None),       # 17 * 16: a prime factor above 13
+    (17 * 8, 0, 8, False, None),
+    (8192, 1, 1, False, None),       # longer than a pass tile, and so is B = 8192
+    (8192, 1, 64, True, True),       # 64 x 128: phase order only
+    (65536, 1, 256, True, True),
+    (16384, 0, 128, True, True),
+    (8192, 0, 64, True, False),      # one complex-float pass tile
+    (4096, 1, 64, True, False),
+    (16 * 27 * 5, 1, 36, True, False),
+])
+def test_split_plan_axis_support_at_plan_time(C, L, dbl, A, ok, phased):
+    """An axis that neither one fused pass nor the plan's A x B factors can
+    transform (a prime factor above 13, or a factor longer than one tile)
+    makes buildParallelPlan return NULL, as the reference's other plan
+    rejections do (split_fft.c:252-308), and makeSplitPlan raise 'Cannot
+    generate plan' with the reason.  An axis longer than one pass tile
+    whose factors each fit one gives a plan that runs in the reference's
+    phase order (and refuses to leave it)."""
     comm = C.dm_world_create(2)
     start = C.makePackedDecomp(comm, 3, dm.ptrs([prod_dim(C, ("local", L)), prod_dim(C, ("block", 2, 2)),
                                                  prod_dim(C, ("local", 1))]))
     per = C.extent(start)
     length = 4 * 2 * L + 64
     dat, work = HostBuffers(C, comm, 2, per), HostBuffers(C, comm, 2, length)
-    b = max(q for q in range(1, L + 1) if L % q == 0 and q * q <= L)
-    A, B, Ls = [b, 1, 1], [L // b, 2, 1], [L, 2, 1]
-    ga, gb = oracle.processor_grid(2, A), oracle.processor_grid(2, B)
+    As, B, Ls = [A, 1, 1], [L // A, 2, 1], [L, 2, 1]
+    ga, gb = oracle.processor_grid(2, As), oracle.processor_grid(2, B)
     build = C.buildParallelPlan if dbl else C.buildParallelPlanf
     make = C.makeSplitPlan if dbl else C.makeSplitPlanf
     C.dm_set_error_mode(1)
     try:
         C.dm_clear_error()
-        p = build(0, 0, 0, start, 3, dm.ints(A), dm.ints(B), dm.ints(Ls), dm.ints(ga), dm.ints(gb),
+        p = build(0, 0, 0, start, 3, dm.ints(As), dm.ints(B), dm.ints(Ls), dm.ints(ga), dm.ints(gb),
                   dat.ptr, length, work.ptr)
         assert bool(p) == ok
         assert C.dm_last_error() == b""   # a rejection, not an error
         if p:
+            assert C.dm_split_phase_order(p) == int(phased)
+            if phased:
+                assert C.dm_split_set_phase_order(p, 0) != 0
+                assert b"phase order only" in C.dm_last_error()
+                C.dm_clear_error()
+            else:
+                assert C.dm_split_set_phase_order(p, 1) == 0 and C.dm_split_phase_order(p) == 1
+                assert C.dm_split_set_phase_order(p, 0) == 0 and C.dm_split_phase_order(p) == 0
             C.freeSplitPlan(p)
-        p = make(0, start, dat.ptr, length, work.ptr)
-        assert bool(p) == ok
-        if ok:
-            C.freeSplitPlan(p)
-        else:
-            msg = C.dm_last_error()
-            assert b"Cannot generate plan" in msg and b"axis 0" in msg, msg
+        if A > 1 and A * A <= L and not any(L % q == 0 and A < q and q * q <= L for q in range(2, L)):
+            # A is makeSplitPlan's own choice (largest divisor not above sqrt(L))
+            p = make(0, start, dat.ptr, length, work.ptr)
+            assert bool(p) == ok
+            if ok:
+                C.freeSplitPlan(p)
+            else:
+                msg = C.dm_last_error()
+                assert b"Cannot generate plan" in msg and b"axis 0" in msg, msg
     finally:
         C.dm_clear_error()
         C.dm_set_error_mode(0)
+    dat.free()
+    work.free()
+    C.freeDecomp(start)
+    C.dm_world_free(comm)
+
+
+def test_phase_order_flag(C):
+    """RUN_PHASE_ORDER among buildParallelPlan's flags makes a plan that
+    runs in phase order; without it the fused program is the default."""
+    comm = C.dm_world_create(4)
+    start = C.makePackedDecomp(comm, 3, dm.ptrs([prod_dim(C, ("block", 16, 1)), prod_dim(C, ("block", 16, 2)),
+                                                 prod_dim(C, ("block", 16, 2))]))
+    per = C.extent(start)
+    dat, work = HostBuffers(C, comm, 4, per), HostBuffers(C, comm, 4, 4 * per + 64)
+    A, B, L = [4, 4, 4], [4, 4, 4], [16, 16, 16]
+    ga, gb = oracle.processor_grid(4, A), oracle.processor_grid(4, B)
+    for flags, want in ((0, 0), (dm.RUN_PHASE_ORDER, 1)):
+        p = C.buildParallelPlan(flags, 0, 0, start, 3, dm.ints(A), dm.ints(B), dm.ints(L), dm.ints(ga),
+                                dm.ints(gb), dat.ptr, 4 * per + 64, work.ptr)
+        assert p and C.dm_split_phase_order(p) == want
+        C.freeSplitPlan(p)
     dat.free()
     work.free()
     C.freeDecomp(start)
