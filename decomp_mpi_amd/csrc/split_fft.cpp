@@ -487,19 +487,35 @@ static bool compile_plan(SplitPlan plan) {
 // B -> A, conjugate TWIDDLE, FFT_1, RESHAPE_1 A -> start).  Each phase is
 // timed into plan->times under its own name.
 
-// Stage passes over the sub-axes `subs` of a split layout, in place on buf,
-// fastest sub-axis first.
-static bool compile_stage(SplitPlan plan, const dm::Side &side, std::vector<int> subs, bool inverse, void *buf,
-                          dm_fft_program *prog) {
+// Stage passes, in place on buf: for each axis d in `axes` the DFT along
+// one factor of d -- the z part (B_d points) for stage 1, the y part (A_d
+// points) for stage 2 -- on a view of the layout in which only axis d is
+// split, so that a tile's lines can run along a whole other axis (long
+// contiguous rows, many lines per tile).  Fastest axis first.
+static bool compile_stage(SplitPlan plan, const dm::HostLayout &h, std::vector<int> axes, bool stage1,
+                          bool inverse, void *buf, i64 slot, dm_fft_program *prog) {
+  dm_split_ext *x = plan->ext;
   prog->passes.clear();
   prog->mem = nullptr;
   prog->inverse = inverse ? 1 : 0;
-  if (subs.empty()) return true;
-  std::stable_sort(subs.begin(), subs.end(), [&](int a, int b) { return side.h.m[a] < side.h.m[b]; });
+  if (axes.empty()) return true;
+  std::stable_sort(axes.begin(), axes.end(), [&](int a, int b) { return h.m[a] < h.m[b]; });
   dm::TableBuilder tb;
-  std::vector<dm::PendingPass> pp(subs.size());
-  for (size_t i = 0; i < subs.size(); i++)
-    if (!dm::plan_pass(plan->comm, side, side, subs[i], plan->ext->dbl, inverse, buf, buf, tb, &pp[i])) return false;
+  std::vector<dm::PendingPass> pp(axes.size());
+  for (size_t i = 0; i < axes.size(); i++) {
+    const int d = axes[i];
+    std::vector<int> f(x->ndim, 1);
+    f[d] = x->A[d];
+    dm::HostLayout sh;
+    std::string why;
+    if (!dm::split_layout(h, f, &sh, &why)) {
+      raise_error(plan->comm, "split FFT: axis %d of a stage layout: %s", d, why.c_str());
+      return false;
+    }
+    dm::Side side = dm::make_side(sh, slot);
+    const int sub = stage1 ? 2 * d + 1 : 2 * d;
+    if (!dm::plan_pass(plan->comm, side, side, sub, x->dbl, inverse, buf, buf, tb, &pp[i])) return false;
+  }
   prog->mem = dm::finalize_passes(plan->comm, tb, pp, &prog->passes);
   return prog->mem != nullptr;
 }
@@ -517,33 +533,23 @@ static bool compile_phased(SplitPlan plan) {
   char *w0 = bw.base + ow;
   char *w1 = w0 + (size_t)plan->extent * es;
   const bool two = !x->use_first;
-  std::string why;
-  dm::HostLayout hA = dm::layout_from_decomp(plan->A), sA;
+  dm::HostLayout hA = dm::layout_from_decomp(plan->A);
   if (!dm::layout_usable(comm, "A", hA, (size_t)plan->extent)) return false;
-  if (!dm::split_layout(hA, x->A, &sA, &why)) {
-    raise_error(comm, "split FFT: A decomposition: %s", why.c_str());
-    return false;
-  }
-  dm::Side SA = dm::make_side(sA, (i64)(bw.slot / es));
-  std::vector<int> z_subs, y_subs;
+  const i64 wslot = (i64)(bw.slot / es);
+  std::vector<int> z_axes, y_axes;
   for (int d = 0; d < nd; d++) {
-    if (x->B[d] > 1) z_subs.push_back(2 * d + 1);
-    if (x->A[d] > 1) y_subs.push_back(2 * d);
+    if (x->B[d] > 1) z_axes.push_back(d);
+    if (x->A[d] > 1) y_axes.push_back(d);
   }
   void *st1_buf = two ? w1 : w0;
-  if (!compile_stage(plan, SA, z_subs, false, st1_buf, &ph.st1_fwd) ||
-      !compile_stage(plan, SA, z_subs, true, st1_buf, &ph.st1_bwd))
+  if (!compile_stage(plan, hA, z_axes, true, false, st1_buf, wslot, &ph.st1_fwd) ||
+      !compile_stage(plan, hA, z_axes, true, true, st1_buf, wslot, &ph.st1_bwd))
     return false;
   if (two) {
-    dm::HostLayout hB = dm::layout_from_decomp(plan->B), sB;
+    dm::HostLayout hB = dm::layout_from_decomp(plan->B);
     if (!dm::layout_usable(comm, "B", hB, (size_t)plan->extent)) return false;
-    if (!dm::split_layout(hB, x->A, &sB, &why)) {
-      raise_error(comm, "split FFT: B decomposition: %s", why.c_str());
-      return false;
-    }
-    dm::Side SB = dm::make_side(sB, (i64)(bw.slot / es));
-    if (!compile_stage(plan, SB, y_subs, false, w0, &ph.st2_fwd) ||
-        !compile_stage(plan, SB, y_subs, true, w0, &ph.st2_bwd))
+    if (!compile_stage(plan, hB, y_axes, false, false, w0, wslot, &ph.st2_fwd) ||
+        !compile_stage(plan, hB, y_axes, false, true, w0, wslot, &ph.st2_bwd))
       return false;
     if (plan->A_to_B == nullptr) {  // single second stage: the reference has no A -> B plan
       MPI_Datatype base = x->dbl ? &dm_type_c_double_complex : &dm_type_c_float_complex;
