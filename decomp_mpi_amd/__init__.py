@@ -81,6 +81,7 @@ _SIGS = {
     "dm_world_create": (P, [I]), "dm_comm_view": (P, [P, I]), "dm_world_free": (None, [P]),
     "dm_comm_rank": (I, [P, IP]), "dm_comm_size": (I, [P, IP]),
     "dm_type_contiguous": (I, [I, P, ctypes.POINTER(P)]), "dm_type_free": (I, [ctypes.POINTER(P)]),
+    "dm_type_get_extent": (I, [P, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
     "dm_type_size": (I, [P, IP]),
     # runtime extensions (dm_runtime.h)
     "dm_malloc": (P, [P, SZ]), "dm_free": (I, [P]), "dm_register": (I, [P, P, SZ]),
@@ -108,6 +109,9 @@ _SIGS = {
     # axis / decomp
     "makeAxisDecomp": (P, [P, P, I, I]), "freeAxisDecomp": (None, [P]), "isLocal": (I, [P, I]),
     "makeIndexList": (P, [P, P]), "makeGlobalIndex": (P, [P]), "freeIndexList": (None, [P]),
+    "makeIOType": (None, [ctypes.POINTER(P), P, P]), "expandIOType": (I, [ctypes.POINTER(P), ctypes.c_long, P, P, I]),
+    "dm_type_view": (ctypes.c_longlong, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong]),
+    "dm_file_write": (I, [ctypes.c_char_p, P, P, P]), "dm_file_read": (I, [ctypes.c_char_p, P, P, P]),
     "freeIndex": (None, [P]),
     "makeDecomp": (P, [I, P]), "makePackedDecomp": (P, [P, I, P]), "freeDecomp": (None, [P]),
     "extent": (I, [P]), "compatible": (I, [P, P]), "equivalent": (I, [P, P]), "myRank": (I, [P]),
@@ -126,6 +130,7 @@ _SIGS = {
     "makeParallelPlanf": (P, [P, IP, P, I, P]),
     "getTransformed": (P, [P]), "runSplitPlan": (None, [P, I]), "freeSplitPlan": (None, [P]),
     "dm_split_set_phase_order": (I, [P, I]), "dm_split_phase_order": (I, [P]),
+    "dm_split_set_output": (I, [P, P]),
     "makeInterleavedDecomp": (P, [I, P, IP, IP, IP]), "makeBlockedDecomp": (P, [I, P, IP, IP, IP]),
     "gcd": (ctypes.c_uint, [ctypes.c_uint, ctypes.c_uint]), "decomp_scale": (None, [P, P, P, I]),
     "makeProcessorGrid": (IP, [P, I, IP]), "visitData": (None, [P, P, P]),

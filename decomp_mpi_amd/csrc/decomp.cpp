@@ -106,13 +106,85 @@ extern "C" Index makeGlobalIndex(AxisDecomp mine) {
   return res;
 }
 
-extern "C" int expandIOType(MPI_Datatype *, MPI_Aint, MPI_Datatype, AxisDecomp mine, int) {
-  raise_error(mine ? mine->comm : nullptr, "MPI-IO file views are not supported by this library");
-  return 0;
+// reshape.c:158-172.  The file view of one axis: for every coordinate g
+// this rank holds along it, ascending, a copy of src's view placed at
+// g * extent * stride bytes (src resized to extent * stride, then indexed
+// by the rank's global index runs).  Returns stride * coord_size, the
+// stride of the next (slower) axis.
+extern "C" int expandIOType(MPI_Datatype *result, MPI_Aint extent, MPI_Datatype src, AxisDecomp mine, int stride) {
+  if (result == nullptr || src == nullptr || mine == nullptr) {
+    raise_error(mine ? mine->comm : nullptr, "expandIOType: NULL argument");
+    return 0;
+  }
+  std::vector<long long> inner = src->view;
+  if (inner.empty()) inner.push_back(0);
+  dm_datatype *t = new dm_datatype;
+  t->predefined = 0;
+  t->elem = src->elem ? src->elem : src->size;
+  const long long step = (long long)extent * stride;
+  long long top = 0;
+  for (int g = 0; g < mine->decomp->coord_size; g++) {
+    if (!isLocal(mine, g)) continue;
+    for (long long o : inner) t->view.push_back(g * step + o);
+    top = (g + 1) * step;
+  }
+  t->count = (int)t->view.size();
+  t->size = t->view.size() * t->elem;
+  t->extent = top;
+  t->true_lb = 0;
+  t->true_extent = top;
+  *result = t;
+  return stride * mine->decomp->coord_size;
 }
 
-extern "C" void makeIOType(MPI_Datatype *, MPI_Datatype, Decomp d) {
-  raise_error(d ? d->comm : nullptr, "MPI-IO file views are not supported by this library");
+// reshape.c:494-521.  The view of a whole decomposition: the calling rank's
+// elements at their global row-major positions (last axis fastest), in the
+// order of its loops over its coordinates (first axis outermost), spanning
+// the whole file; empty (extent 0) on ranks the decomposition does not use.
+extern "C" void makeIOType(MPI_Datatype *result, MPI_Datatype src, Decomp d) {
+  if (result == nullptr || src == nullptr || d == nullptr) {
+    raise_error(d ? d->comm : nullptr, "makeIOType: NULL argument");
+    return;
+  }
+  dm_aint_t lb = 0, extent = 0;
+  dm_type_get_extent(src, &lb, &extent);
+  dm_datatype_t cur = nullptr;
+  dm_type_dup(src, &cur);
+  cur->view.assign(1, 0);
+  cur->elem = src->elem ? src->elem : src->size;
+  int stride = 1;
+  if (d->comm->rank < d->proc_used) {
+    for (int i = d->ndim - 1; i >= 0; i--) {
+      dm_datatype_t next = nullptr;
+      stride = expandIOType(&next, extent, cur, d->dims[i], stride);
+      dm_type_free(&cur);
+      if (next == nullptr) return;
+      cur = next;
+    }
+    cur->extent = (ptrdiff_t)stride * extent;
+  } else {
+    cur->view.clear();
+    cur->count = 0;
+    cur->size = 0;
+    cur->extent = 0;
+  }
+  cur->true_extent = cur->extent;
+  *result = cur;
+}
+
+extern "C" long long dm_type_view(MPI_Datatype t, long long *offsets, long long max) {
+  if (t == nullptr) return -1;
+  if (t->view.empty()) {
+    if (t->predefined || t->count == 1) {
+      if (offsets != nullptr && max > 0) offsets[0] = 0;
+      return 1;
+    }
+    return 0;
+  }
+  const long long n = (long long)t->view.size();
+  if (offsets != nullptr)
+    for (long long i = 0; i < n && i < max; i++) offsets[i] = t->view[i];
+  return n;
 }
 
 // ---------------------------------------------------------------- Decomp
@@ -321,6 +393,31 @@ bool layout_rank_map_ok(Decomp d, std::string *why) {
     }
   }
   return true;
+}
+
+// FNV-1a over the global description of d: axes (sizes, rank strides, the
+// owner and offset of every coordinate) and every rank's memory strides.
+long long decomp_fingerprint(Decomp d) {
+  unsigned long long h = 1469598103934665603ULL;
+  auto mix = [&](long long v) {
+    for (int k = 0; k < 8; k++) {
+      h ^= (unsigned long long)((v >> (8 * k)) & 0xff);
+      h *= 1099511628211ULL;
+    }
+  };
+  if (d == nullptr) return 0;
+  mix(d->ndim);
+  mix(d->proc_used);
+  for (int i = 0; i < d->ndim; i++) {
+    AxisDecomp a = d->dims[i];
+    mix(a->decomp->coord_size);
+    mix(a->decomp->proc_size);
+    mix(a->rank_stride);
+    for (int v : a->ext->rank_of) mix(v);
+    for (int v : a->ext->off_of) mix(v);
+  }
+  for (i64 v : d->ext->strides) mix(v);
+  return (long long)h;
 }
 
 HostLayout layout_from_decomp(Decomp d) {

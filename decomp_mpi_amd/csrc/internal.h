@@ -39,6 +39,10 @@ struct dm_datatype {
   int count;          // elements of a message description (1 for base types)
   ptrdiff_t true_lb;  // true lower bound (bytes) of a message description
   ptrdiff_t true_extent;
+  // File views (makeIOType / expandIOType): byte offsets of the base
+  // elements, in order, and the base element size; empty for other types.
+  std::vector<long long> view;
+  size_t elem = 0;
 };
 
 // Per-axis global view: the DimDecomp evaluated at every coordinate plus
@@ -106,6 +110,17 @@ bool resolve_rank(const void *p, Buffer *buf, int *rank, size_t *offset);
 // all ranks wait again.
 int spmd_rank();
 void spmd_collective(const std::function<void()> &fn);
+// The same, after checking that every rank's argument signature equals
+// rank 0's; when they differ nothing runs and every rank raises an error.
+bool spmd_collective_checked(const std::vector<long long> &sig, const char *what,
+                             const std::function<void()> &fn);
+// Argument signatures: a pointer as (buffer base, slot rank, offset) or
+// (-1, -1, address) when it is no registered buffer's; a decomposition's
+// global layout as a hash.
+void sig_pointer(std::vector<long long> *sig, const void *p);
+// All ranks of the calling thread's session wait for each other.
+void spmd_barrier_internal();
+long long decomp_fingerprint(Decomp d);
 
 // ---- layouts (layout.cpp)
 // A decomposition flattened into per-axis lookup tables over global

@@ -396,7 +396,12 @@ static void run_reshape(MPI_Comm comm, ReshapePlan plan, void *src, void *dest);
 // pointers are the buffers' slot-0 pointers) moves every rank's share.
 extern "C" void runReshapePlan(MPI_Comm comm, ReshapePlan plan, void *src, void *dest) {
   if (dm::spmd_rank() >= 0) {
-    dm::spmd_collective([&] { run_reshape(comm, plan, src, dest); });
+    std::vector<long long> sig;
+    sig.push_back(plan ? dm::decomp_fingerprint(plan->src) : 0);
+    sig.push_back(plan ? dm::decomp_fingerprint(plan->dest) : 0);
+    dm::sig_pointer(&sig, src);
+    dm::sig_pointer(&sig, dest);
+    dm::spmd_collective_checked(sig, "runReshapePlan", [&] { run_reshape(comm, plan, src, dest); });
     return;
   }
   run_reshape(comm, plan, src, dest);

@@ -212,6 +212,7 @@ struct dm_split_ext {
   dm_fft_program fwd, bwd;
   bool phased = false;      // run in the reference's phase order
   bool fused_ok = true;     // every axis fits one pass (else phased only)
+  bool custom_out = false;  // out is the caller's (dm_split_set_output)
   dm_phased_program ph;
 };
 
@@ -956,7 +957,32 @@ static double time_plan(SplitPlan p, int reps) {
   return ok ? ms / reps : 1e30;
 }
 
+static SplitPlan autotune_search(int dbl, Decomp start, void *dat, int length, void *workspace);
+
+// Collective in a dm_spmd_run session (as the reference's search is, with
+// its barriers and all-reduces): rank 0 searches and times, every rank then
+// plans the chosen factors with its own view.
 static SplitPlan autotune(int dbl, Decomp start, void *dat, int length, void *workspace) {
+  if (dm::spmd_rank() < 0) return autotune_search(dbl, start, dat, length, workspace);
+  static std::vector<int> chosen;  // rank 0 writes it between the rendezvous
+  std::vector<long long> sig = {dbl, length, dm::decomp_fingerprint(start)};
+  dm::sig_pointer(&sig, dat);
+  dm::sig_pointer(&sig, workspace);
+  if (!dm::spmd_collective_checked(sig, "autotuneParallelPlan", [&] {
+        chosen.clear();
+        SplitPlan best = autotune_search(dbl, start, dat, length, workspace);
+        if (best == nullptr) return;
+        for (int i = 0; i < best->ext->ndim; i++) chosen.push_back(best->ext->B[i]);
+        freeSplitPlan(best);
+      }))
+    return nullptr;
+  std::vector<int> B = chosen;
+  dm::spmd_barrier_internal();  // every rank has copied the choice before the next search may write it
+  if (B.empty()) return nullptr;
+  return propose(dbl, PREFER_STAGE1, start, B.data(), dat, length, workspace);
+}
+
+static SplitPlan autotune_search(int dbl, Decomp start, void *dat, int length, void *workspace) {
   int ndim = start->ndim;
   std::vector<int> L(ndim);
   for (int i = 0; i < ndim; i++) L[i] = sizeDimDecomp(start->dims[i]->decomp);
@@ -1085,7 +1111,19 @@ static void run_split(SplitPlan plan, int direction);
 // (built on slot-0 pointers) runs for every rank.
 extern "C" void runSplitPlan(SplitPlan plan, int direction) {
   if (dm::spmd_rank() >= 0) {
-    dm::spmd_collective([&] { run_split(plan, direction); });
+    std::vector<long long> sig = {direction};
+    if (plan != nullptr && plan->ext != nullptr) {
+      dm_split_ext *x = plan->ext;
+      sig.insert(sig.end(), x->A.begin(), x->A.end());
+      sig.push_back(x->dbl);
+      sig.push_back(x->phased);
+      sig.push_back(plan->extent);
+      sig.push_back(dm::decomp_fingerprint(x->start));
+      sig.push_back(dm::decomp_fingerprint(x->out));
+      dm::sig_pointer(&sig, plan->src);
+      dm::sig_pointer(&sig, plan->work);
+    }
+    dm::spmd_collective_checked(sig, "runSplitPlan", [&] { run_split(plan, direction); });
     return;
   }
   run_split(plan, direction);
@@ -1111,8 +1149,37 @@ static void run_split(SplitPlan plan, int direction) {
   else run_program(plan, plan->ext->bwd, FFT_2, FFT_1);
 }
 
+extern "C" int dm_split_set_output(SplitPlan plan, Decomp out) {
+  if (plan == nullptr || plan->ext == nullptr || out == nullptr) return DM_ERR_ARG;
+  dm_split_ext *x = plan->ext;
+  if (compatible(out, x->start) != COMPATIBLE) {
+    raise_error(plan->comm, "dm_split_set_output: the output decomposition is not compatible with the start");
+    return DM_ERR_ARG;
+  }
+  if (!x->fused_ok) {
+    raise_error(plan->comm, "dm_split_set_output: this plan runs in phase order only, which writes the B layout");
+    return DM_ERR_ARG;
+  }
+  std::string why;
+  if (!dm::layout_rank_map_ok(out, &why)) {
+    raise_error(plan->comm, "dm_split_set_output: %s", why.c_str());
+    return DM_ERR_ARG;
+  }
+  program_release(x->fwd);
+  program_release(x->bwd);
+  x->compiled = false;
+  x->out = out;
+  x->custom_out = true;
+  x->phased = false;
+  return DM_SUCCESS;
+}
+
 extern "C" int dm_split_set_phase_order(SplitPlan plan, int on) {
   if (plan == nullptr || plan->ext == nullptr) return DM_ERR_ARG;
+  if (on && plan->ext->custom_out) {
+    raise_error(plan->comm, "dm_split_set_phase_order: the phase order writes the B layout, not a chosen output");
+    return DM_ERR_ARG;
+  }
   if (!on && !plan->ext->fused_ok) {
     raise_error(plan->comm, "dm_split_set_phase_order: an axis is longer than one pass tile; the plan runs in "
                             "phase order only");
@@ -1126,7 +1193,10 @@ extern "C" int dm_split_phase_order(SplitPlan plan) {
   return (plan != nullptr && plan->ext != nullptr && plan->ext->phased) ? 1 : 0;
 }
 
-extern "C" Decomp getTransformed(SplitPlan plan) { return plan->B_flipped; }
+extern "C" Decomp getTransformed(SplitPlan plan) {
+  if (plan != nullptr && plan->ext != nullptr && plan->ext->custom_out) return plan->ext->out;
+  return plan->B_flipped;
+}
 
 extern "C" void freeSplitPlan(SplitPlan plan) {
   if (plan == nullptr) return;

@@ -18,7 +18,10 @@ struct Session {
   int waiting = 0;
   long long generation = 0;
   long long sum = 0, result = 0;  // all-reduce scratch
+  double dmax = 0, dresult = 0;
   void *shared = nullptr;         // collective allocation
+  std::vector<std::vector<long long>> sigs;  // collective arguments, per rank
+  bool agree = true;
 };
 
 thread_local Session *tls_session = nullptr;
@@ -56,6 +59,31 @@ void spmd_collective(const std::function<void()> &fn) {
   if (tls_rank == 0) fn();
   arrive(s, [] {});
 }
+
+void spmd_barrier_internal() {
+  if (tls_session != nullptr) arrive(tls_session, [] {});
+}
+
+bool spmd_collective_checked(const std::vector<long long> &sig, const char *what, const std::function<void()> &fn) {
+  Session *s = tls_session;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->sigs[tls_rank] = sig;
+  }
+  arrive(s, [s] {
+    s->agree = true;
+    for (int r = 1; r < s->P; r++) s->agree = s->agree && s->sigs[r] == s->sigs[0];
+  });
+  const bool agree = s->agree;
+  if (agree && tls_rank == 0) fn();
+  arrive(s, [] {});
+  if (!agree)
+    raise_error(s->world->views[tls_rank],
+                "%s: the ranks' arguments differ (each rank passes the same plan and its own slot of the same "
+                "buffers at the same offset)",
+                what);
+  return agree;
+}
 }  // namespace dm
 
 extern "C" int dm_spmd_run(MPI_Comm world, void (*body)(MPI_Comm me, void *arg), void *arg) {
@@ -67,6 +95,7 @@ extern "C" int dm_spmd_run(MPI_Comm world, void (*body)(MPI_Comm me, void *arg),
   Session s;
   s.P = world->world->nranks;
   s.world = world->world;
+  s.sigs.resize(s.P);
   std::vector<std::thread> threads;
   threads.reserve(s.P);
   for (int r = 0; r < s.P; r++)
@@ -97,6 +126,20 @@ extern "C" long long dm_spmd_allreduce_sum(MPI_Comm me, long long value) {
   arrive(s, [s] { s->result = s->sum; });
   long long r = s->result;
   arrive(s, [] {});  // nobody resets sum before everyone has read result
+  return r;
+}
+
+extern "C" double dm_spmd_allreduce_max(MPI_Comm me, double value) {
+  Session *s = session_of(me);
+  if (s == nullptr) return value;
+  arrive(s, [s] { s->dmax = -1e308; });
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    if (value > s->dmax) s->dmax = value;
+  }
+  arrive(s, [s] { s->dresult = s->dmax; });
+  double r = s->dresult;
+  arrive(s, [] {});
   return r;
 }
 

@@ -61,7 +61,7 @@ extern "C" void dm_abort(dm_comm_t comm, int code) {
 // ---------------------------------------------------------------- types
 
 #define DM_BASE_TYPE(name, bytes) \
-  struct dm_datatype name = {bytes, bytes, 1, 1, 0, bytes}
+  struct dm_datatype name = {bytes, bytes, 1, 1, 0, bytes, {}, bytes}
 DM_BASE_TYPE(dm_type_byte, 1);
 DM_BASE_TYPE(dm_type_char, 1);
 DM_BASE_TYPE(dm_type_int, 4);
@@ -81,6 +81,7 @@ extern "C" int dm_type_contiguous(int count, dm_datatype_t old, dm_datatype_t *r
   t->count = 1;
   t->true_lb = 0;
   t->true_extent = t->extent;
+  t->elem = t->size;
   *result = t;
   return DM_SUCCESS;
 }
@@ -231,6 +232,21 @@ bool resolve(const void *p, Buffer *buf, size_t *offset) {
   *buf = b;
   *offset = a - it->first;
   return true;
+}
+
+void sig_pointer(std::vector<long long> *sig, const void *p) {
+  Buffer b;
+  int r;
+  size_t off;
+  if (resolve_rank(p, &b, &r, &off)) {
+    sig->push_back((long long)(uintptr_t)b.base);
+    sig->push_back((long long)off);
+    sig->push_back(r == spmd_rank() ? 1 : 0);  // each rank passes its own slot
+  } else {
+    sig->push_back(-1);
+    sig->push_back(-1);
+    sig->push_back((long long)(uintptr_t)p);
+  }
 }
 
 bool resolve_rank(const void *p, Buffer *buf, int *rank, size_t *offset) {

@@ -69,7 +69,11 @@ struct axis_decomp {
 };
 
 Index *makeIndexList(AxisDecomp local, DimDecomp remote);
+/** reshape.c:137-148: runs of the global coordinates local to this rank. */
 Index makeGlobalIndex(AxisDecomp local);
+/** reshape.c:158-172: the view of one axis -- src placed at g * extent *
+ *  stride bytes for every local coordinate g, ascending; returns
+ *  stride * coord_size. */
 int expandIOType(MPI_Datatype *result, MPI_Aint extent, MPI_Datatype src,
                  AxisDecomp mine, int stride);
 AxisDecomp makeAxisDecomp(MPI_Comm comm, DimDecomp d, int stride,
@@ -156,8 +160,24 @@ void runReshapePlan(MPI_Comm comm, ReshapePlan plan, void *src, void *dest);
  *  else 1 after a message on stderr (reshape.h:293). */
 int checkReshapePlanBounds(ReshapePlan plan, int type_size, int src_len,
                            int dest_len);
-/** MPI-IO file views are not part of this library; raises an error. */
+/** reshape.c:494-521: the calling rank's file view of d over the global
+ *  row-major file (last axis fastest): its elements' byte offsets, in the
+ *  order of its loops over its coordinates (first axis outermost), in a
+ *  datatype whose extent is the whole file (0 on ranks d does not use).
+ *  dm_type_view reads the offsets; dm_file_write / dm_file_read move the
+ *  whole distributed array to and from such a file. */
 void makeIOType(MPI_Datatype *result, MPI_Datatype src, Decomp d);
+/** No counterpart in the reference (its view is an MPI datatype): number of
+ *  base elements of type t's view, their byte offsets written to
+ *  offsets[0..max). */
+long long dm_type_view(MPI_Datatype t, long long *offsets, long long max);
+/** No counterpart in the reference, whose drivers write through MPI-IO with
+ *  makeIOType's views (split_plan_test.c:160-200).  Collective: the whole
+ *  distributed array buf (decomposition d, elements of type base) to or
+ *  from the file at path as raw native-endian elements in global row-major
+ *  order (the reference's "native" representation).  0 on success. */
+int dm_file_write(const char *path, MPI_Datatype base, Decomp d, const void *buf);
+int dm_file_read(const char *path, MPI_Datatype base, Decomp d, void *buf);
 
 #ifdef __cplusplus
 }

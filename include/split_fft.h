@@ -115,6 +115,17 @@ SplitPlan makeParallelPlanf(Decomp start, int *initial_factor, fftwf_complex *da
 int dm_split_set_phase_order(SplitPlan plan, int on);
 /** 1 when the plan runs in phase order. */
 int dm_split_phase_order(SplitPlan plan);
+/** No counterpart in the reference.  The forward result goes to
+ *  work[0:extent) in the caller's decomposition `out` (compatible with the
+ *  start, every coordinate owned, each rank's block within extent elements)
+ *  instead of the B layout, and the backward reads it from there;
+ *  getTransformed then returns `out`, which the plan aliases.  For example
+ *  FFTW-MPI's FFTW_MPI_TRANSPOSED_OUT layout of an n0 x n1 x n2 transform
+ *  over P ranks -- axis 1 in blocks over the ranks, local order [k1][k0][k2]
+ *  -- is makeDecomp of makeAxisDecomp(Local(n0), n2, 1),
+ *  makeAxisDecomp(Block(n1, P), n0 * n2, 1), makeAxisDecomp(Local(n2), 1, 1)
+ *  (fftw_test.c:277-281).  Fused plans only.  Returns 0 or an error code. */
+int dm_split_set_output(SplitPlan plan, Decomp out);
 
 /** The decomposition of the forward result: frequency -> rank and offset
  *  in work (split_fft.c:615-621; NULL for single-stage plans). */

@@ -24,12 +24,14 @@ def test_c_program_plans(tmp_path, complex_first):
     assert "A=(8,8,8)" in out
 
 
-def test_spmd_driver_compiles(tmp_path):
-    """The MPI-style driver (one thread per rank) builds against the headers."""
-    exe = tmp_path / "spmd_test_c"
+@pytest.mark.parametrize("name", ["spmd_test_c", "split_plan_test_c", "plan_compare_c"])
+def test_spmd_driver_compiles(tmp_path, name):
+    """The MPI-style drivers (one thread per rank) build against the
+    headers without warnings."""
+    exe = tmp_path / name
     subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-pthread", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "c_api", "spmd_test_c.c"), "-o", str(exe),
-                    "-L", LIBDIR, "-ldecomp_mpi_amd", "-Wl,-rpath," + LIBDIR], check=True)
+                    os.path.join(ROOT, "tests", "c_api", name + ".c"), "-o", str(exe),
+                    "-L", LIBDIR, "-ldecomp_mpi_amd", "-Wl,-rpath," + LIBDIR, "-lm"], check=True)
 
 
 def test_spmd_session_collectives():
@@ -59,3 +61,48 @@ def test_spmd_session_collectives():
     assert C.dm_spmd_run(world, cb, None) == 0
     C.dm_world_free(world)
     assert seen == {r: (P * (P + 1) // 2, P) for r in range(P)}
+
+
+def test_spmd_collective_arguments_must_agree():
+    """Inside a dm_spmd_run session a collective whose ranks pass different
+    arguments (here: different offsets into their slots) runs nothing and
+    raises an error on every rank (no device involved: the check comes
+    first)."""
+    import ctypes
+    import numpy as np
+    import decomp_mpi_amd as dm
+    C = dm.lib()
+    BODY = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
+    C.dm_spmd_run.argtypes = [ctypes.c_void_p, BODY, ctypes.c_void_p]
+    P = 3
+    world = C.dm_world_create(P)
+    slot = 4096
+    mem = np.zeros(P * slot * 2 // 8, np.float64)
+    base = mem.ctypes.data
+    assert C.dm_register(world, base, slot) == 0
+    msgs = {}
+
+    def body(me, arg):
+        r = ctypes.c_int()
+        C.dm_comm_rank(me, ctypes.byref(r))
+        d = C.makePackedDecomp(me, 1, dm.ptrs([C.makeBlockDecomp(12, P)]))
+        plan = C.makeReshapePlan(me, dm.mpi_type("int"), d, d)
+        mine = base + r.value * slot
+        C.dm_clear_error()
+        # rank 1 passes its destination 16 bytes further in than the others
+        C.runReshapePlan(me, plan, mine, mine + 2048 + (16 if r.value == 1 else 0))
+        msgs[r.value] = C.dm_last_error()
+        C.freeReshapePlan(plan)
+        C.freeDecomp(d)
+
+    C.dm_set_error_mode(1)
+    try:
+        cb = BODY(body)
+        assert C.dm_spmd_run(world, cb, None) == 0
+    finally:
+        C.dm_set_error_mode(0)
+    C.dm_unregister(base)
+    C.dm_world_free(world)
+    assert sorted(msgs) == list(range(P))
+    for r in range(P):
+        assert b"runReshapePlan: the ranks' arguments differ" in msgs[r], msgs

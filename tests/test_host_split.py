@@ -250,3 +250,52 @@ def test_phase_order_flag(C):
     work.free()
     C.freeDecomp(start)
     C.dm_world_free(comm)
+
+
+IO_CASES = [
+    (6, [("block", 10, 2), ("rotate", 3, ("block", 7, 3)), ("local", 5)], None, None),
+    (4, [("interleave", ("block", 4, 2), 3), ("block", 9, 2)], None, None),
+    (8, [("block", 6, 2), ("local", 4), ("frev", 2, 4, ("block", 8, 2))], None, None),
+    (6, [("block", 8, 2), ("local", 6), ("block", 10, 3)], [1, 1, 2], [1, 8, 48]),
+    (5, [("block", 6, 2), ("local", 4)], None, None),     # ranks 2..4 hold nothing
+]
+
+
+@pytest.mark.parametrize("P,descs,rs,strides", IO_CASES)
+def test_io_type_view_is_the_row_major_file_view(C, P, descs, rs, strides):
+    """makeIOType (reshape.c:494-521): rank r's view lists, in the order of
+    its loops over its coordinates (first axis outermost, ascending), the
+    byte offsets of its elements in the global row-major file; the type
+    spans the whole file, and is empty on ranks the decomposition leaves
+    out.  expandIOType returns the next axis' stride (reshape.c:158-172)."""
+    from helpers import make_pair
+    shape = [rm.coord_size(d) for d in descs]
+    es = 16
+    world = C.dm_world_create(P)
+    _, od = make_pair(C, world, P, descs, rank_strides=rs, strides=strides)
+    for r in range(P):
+        view = C.dm_comm_view(world, r)
+        pd, _ = make_pair(C, view, P, descs, rank_strides=rs, strides=strides)
+        t = ctypes.c_void_p()
+        C.makeIOType(ctypes.byref(t), dm.mpi_type("c_double_complex"), pd)
+        n = C.dm_type_view(t, None, 0)
+        offs = (ctypes.c_longlong * max(n, 1))()
+        assert C.dm_type_view(t, offs, n) == n
+        want = []
+        for g in np.ndindex(*shape):
+            rank, _ = od.owner(g)
+            if rank == r:
+                want.append(int(np.ravel_multi_index(g, shape)) * es)
+        assert list(offs)[:n] == want
+        ext = ctypes.c_long()
+        C.dm_type_get_extent(t, None, ctypes.byref(ext))
+        assert ext.value == (int(np.prod(shape)) * es if r < od.proc_used else 0)
+        C.dm_type_free(ctypes.byref(t))
+        # the last axis alone: stride 1 -> returns its length
+        ax = dm.decomp(pd).dims[len(shape) - 1]
+        t2 = ctypes.c_void_p()
+        assert C.expandIOType(ctypes.byref(t2), es, dm.mpi_type("c_double_complex"),
+                              ctypes.cast(ax, ctypes.c_void_p).value, 1) == shape[-1]
+        C.dm_type_free(ctypes.byref(t2))
+        C.freeDecomp(pd)
+    C.dm_world_free(world)
