@@ -41,6 +41,7 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.or_split_extent.restype = ctypes.c_long
+        _lib.or_message.restype = ctypes.c_long
     return _lib
 
 
@@ -155,6 +156,16 @@ def reshape(src, dst, src_buf, dst_buf, elem):
                           ctypes.c_long(src_buf.shape[1]), _ip(dst_buf), ctypes.c_long(dst_buf.shape[1]))
     if rc != 0:
         raise RuntimeError(f"oracle reshape: unmatched messages (code {rc})")
+
+
+def message(src, dst, me, peer, send):
+    """Element offsets of rank me's message to (send) or from rank peer,
+    in the order the reference's nested datatype lists them, or None."""
+    cap = int(max(src.extents.max(), dst.extents.max())) + 1
+    out = np.zeros(cap, dtype=np.int64)
+    n = lib().or_message(src.P, src.ndim, *src.args(), *dst.args(), ctypes.c_int(me), ctypes.c_int(peer),
+                         ctypes.c_int(1 if send else 0), _ip(out), ctypes.c_long(cap))
+    return None if n < 0 else out[:n].copy()
 
 
 def processor_grid(P, tasks):

@@ -248,6 +248,29 @@ static long message(const Dec *mine, int me, const Dec *remote, int peer, const 
   return total;
 }
 
+/* The message of rank me (of `mine`) to or from rank peer (of `remote`)
+ * as the reference's nested datatype lists it (typelist.c:74-119 with the
+ * axis order of reshape.c:394): element offsets into out (at most max),
+ * in the order the sender or receiver sorts by its own view of the source
+ * strides.  send: mine = src, remote = dst; else mine = dst, remote = src.
+ * Returns the element count, or -1 when there is no such message. */
+long or_message(int P, int ndim, const int *specs_s, const int *off_s, const int *rs_s, int packed_s,
+                const long *expl_s, const int *specs_d, const int *off_d, const int *rs_d, int packed_d,
+                const long *expl_d, int me, int peer, int send, long *out, long max) {
+  Dec S, D;
+  dec_init(&S, P, ndim, specs_s, off_s, rs_s, packed_s, expl_s);
+  dec_init(&D, P, ndim, specs_d, off_d, rs_d, packed_d, expl_d);
+  int order[OR_MAXD];
+  sort_order(&S, me, order);
+  long *o = NULL;
+  long n = send ? message(&S, me, &D, peer, order, &o) : message(&D, me, &S, peer, order, &o);
+  for (long k = 0; k < n && k < max; k++) out[k] = o[k];
+  free(o);
+  dec_free(&S);
+  dec_free(&D);
+  return n;
+}
+
 /* runReshapePlan restated over all ranks: every (S, D) message pair is
  * matched element by element.  Returns 0, or a negative code when a send
  * has no matching receive or the counts differ (where MPI would hang or

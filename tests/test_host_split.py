@@ -299,3 +299,58 @@ def test_io_type_view_is_the_row_major_file_view(C, P, descs, rs, strides):
         C.dm_type_free(ctypes.byref(t2))
         C.freeDecomp(pd)
     C.dm_world_free(world)
+
+
+class DatatypeHead(ctypes.Structure):
+    """The leading fields of this library's datatype (dm_mpi.h's
+    dm_datatype): bytes, extent, predefined, count, true lower bound and
+    true extent."""
+    _fields_ = [("size", ctypes.c_size_t), ("extent", ctypes.c_long), ("predefined", ctypes.c_int),
+                ("count", ctypes.c_int), ("true_lb", ctypes.c_long), ("true_extent", ctypes.c_long)]
+
+
+MSG_CASES = [
+    (4, [("local", 8), ("block", 8, 2), ("block", 8, 2)], [("block", 8, 2), ("local", 8), ("block", 8, 2)]),
+    (6, [("block", 12, 3), ("rotate", 2, ("block", 10, 2)), ("local", 4)],
+     [("interleave", ("block", 4, 2), 3), ("local", 10), ("block", 4, 3)]),
+    (6, [("block", 10, 2), ("local", 6), ("block", 9, 3)], [("single", 10, 0, 1), ("block", 6, 3), ("local", 9)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(MSG_CASES)))
+def test_plan_message_lists_match_oracle(C, case):
+    """makeReshapePlan's send and receive lists (reshape.c:394-420 built by
+    expandList, typelist.c:74-119): for every rank, one entry per peer the
+    oracle's literal message pairing has a message with, carrying that
+    message's element count, true lower bound and true extent in bytes."""
+    from helpers import make_pair
+    P, sd, dd = MSG_CASES[case]
+    es = 8
+    world = C.dm_world_create(P)
+    _, os_ = make_pair(C, world, P, sd)
+    _, od = make_pair(C, world, P, dd)
+    for r in range(P):
+        view = C.dm_comm_view(world, r)
+        ps, _ = make_pair(C, view, P, sd)
+        pd, _ = make_pair(C, view, P, dd)
+        plan = C.makeReshapePlan(view, dm.mpi_type("long"), ps, pd)
+        assert plan
+        rp = dm.reshape_plan(plan)
+        for send, lst in ((True, rp.sends), (False, rp.recvs)):
+            got = {}
+            if lst:
+                tl = lst.contents
+                for k in range(tl.count):
+                    e = tl.list[k]
+                    h = ctypes.cast(e.type, ctypes.POINTER(DatatypeHead)).contents
+                    got[e.rank] = (h.count, h.true_lb, h.true_extent)
+            want = {}
+            for peer in range(P):
+                m = oracle.message(os_, od, r, peer, send)
+                if m is not None:
+                    want[peer] = (len(m), int(m.min()) * es, int(m.max() - m.min() + 1) * es)
+            assert got == want, (r, send, got, want)
+        C.freeReshapePlan(plan)
+        C.freeDecomp(ps)
+        C.freeDecomp(pd)
+    C.dm_world_free(world)
