@@ -181,7 +181,8 @@ def reshape_chain(C, dm, comm, reps=10):
 def batched_bench(C, dm, steps, warmup):
     """256 independent complex-float 3D FFTs of seeded uneven shapes (each
     axis from {32, 48, 64, 96, 128}) in place, one batched launch per axis
-    replayed from a HIP graph; schedule from dm_batch_autotune."""
+    replayed from a HIP graph; grid, tile and radix order from
+    dm_batch_autotune."""
     import ctypes
     from decomp_mpi_amd import workloads
     shapes = workloads.batched_shapes(256, seed=4)
@@ -201,6 +202,7 @@ def batched_bench(C, dm, steps, warmup):
     tune_ms = C.dm_batch_autotune(plan, stream)
     tile, order = ctypes.c_int(), ctypes.c_int()
     C.dm_batch_info(plan, None, None, ctypes.byref(tile), ctypes.byref(order))
+    grid_mode = C.dm_batch_grid(plan)
 
     def upload():
         for x, o in zip(xs, offs):
@@ -234,6 +236,7 @@ def batched_bench(C, dm, steps, warmup):
             "frac_of_peak": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
             "max_rel_l2_vs_fp64": max_rel, "max_abs_err_over_max_vs_fp64": max_abs,
             "schedule": {"tile": ["full", "half"][tile.value], "radix_order": ["large_first", "small_first"][order.value],
+                         "grid": ["one workgroup per tile", "persistent"][grid_mode],
                          "autotune_best_ms": round(tune_ms, 4)}}
 
 
@@ -465,6 +468,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--factors", type=str, default=None,
                     help="B factors b0,b1,b2 for makeParallelPlan instead of the autotuner (profiling runs)")
+    ap.add_argument("--only", choices=["batched", "reshape", "serving"], default=None,
+                    help="run one section alone and print its figures (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the virtual-rank grid stands in for "
@@ -475,6 +480,21 @@ def main():
     C = dm.lib()
     if not C.dm_device_available():
         raise SystemExit("no HIP device")
+    if args.only is not None:
+        if args.only == "batched":
+            res = batched_bench(C, dm, max(args.steps, 5), max(args.warmup, 1))
+            # a plain 1 GiB copy in the same run: the byte counters' calibration
+            comm = C.dm_world_create(1)
+            cb = dm.DistBuffer(comm, 2 << 30)
+            res["copy_1GiB_ms"] = round(C.dm_copy_bench(comm, cb.ptr + (1 << 30), cb.ptr, 1 << 30, 3), 4)
+            cb.free()
+        elif args.only == "serving":
+            res = serving_bench(C, dm)
+        else:
+            comm = C.dm_world_create(P)
+            res = reshape_chain(C, dm, comm)
+        print(json.dumps({"section": args.only, **res}), flush=True)
+        return
     print(f"bench: generating {L}^3 input", file=sys.stderr, flush=True)
     host_x, modes = make_input()
     N = L ** 3

@@ -92,7 +92,8 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
       raise_error(nullptr, "dm_batch_run_noalloc: NULL data pointer for transform %d", i);
       return 1;
     }
-  // the bound form's default schedule (full tiles, large radices first)
+  // the bound form's default schedule (full tiles, large radices first, one
+  // workgroup per tile)
   dm::BatchHost h;
   if (!dm::build_batch_host(count, ndim, shapes, dbl, DM_BATCH_TILE_FULL, DM_BATCH_RADIX_LARGE_FIRST, &h)) {
     raise_error(nullptr, "dm_batch_run_noalloc: unsupported axis length");
@@ -138,7 +139,6 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
     raise_error(nullptr, "dm_batch_run_noalloc: table copy to the scratch array failed: %s", hipGetErrorString(e));
     return 1;
   }
-  const long long resident = dm::batch_resident_groups(dbl, h.primes, h.lds);
   for (int p = 0; p < ndim; p++) {
     dm::BatchLaunch L{};
     L.tiles = (const dm::BatchTile *)(base + o_tiles[p]);
@@ -149,7 +149,7 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
     L.dbl = dbl;
     L.inverse = direction == 1 ? 1 : 0;
     L.primes = h.primes;
-    L.grid = std::min<long long>(L.ntiles, resident);
+    L.grid = L.ntiles;  // one workgroup per tile, as the bound form's default
     L.lds = h.lds;
     if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run_noalloc: batched pass launch"))
       return 1;

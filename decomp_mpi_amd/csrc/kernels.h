@@ -184,6 +184,7 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
 // of T lines, tw_off 0; tw: its N stage-ordered roots as (re, im) doubles.
 void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vector<double> *tw);
 int batch_tile_capacity(int dbl);
+int batch_tile_height(int N, long long inner, int dbl, int tile_choice);
 size_t batch_lds_bytes(int dbl, int T, int N);
 long long batch_resident_groups(int dbl, int primes, size_t lds);
 

@@ -242,7 +242,6 @@ static void plan_release(DmServer s, Req &r) {
 // tile height rule of build_batch_host (as many lines as the tile holds; a
 // column walk never more than its contiguous run).
 static bool plan_request(DmServer s, Req &r) {
-  const int cap = dm::batch_tile_capacity(r.dbl);
   ReqPlan &pl = r.plan;
   pl = ReqPlan();
   pl.npass = r.ndim;
@@ -252,8 +251,7 @@ static bool plan_request(DmServer s, Req &r) {
     for (int e = d + 1; e < r.ndim; e++) inner *= r.shape[e];
     for (int e = 0; e < d; e++) outer *= r.shape[e];
     const int N = r.shape[d];
-    int T = std::max(1, cap / N);
-    if (inner > 1) T = (int)std::min<long long>(T, inner);
+    const int T = dm::batch_tile_height(N, inner, r.dbl, DM_BATCH_TILE_FULL);
     Sched *sc = sched_for(s, N, T, r.dbl);
     if (sc == nullptr) {
       plan_release(s, r);
@@ -419,7 +417,6 @@ static bool launch(DmServer s, Slot &sl) {
     lds = std::max(lds, s->reqs[id].plan.lds);
     primes = primes || s->reqs[id].plan.primes;
   }
-  const long long resident = dm::batch_resident_groups(dbl, primes, lds);
   for (int d = 0; d < nd && ok; d++) {
     dm::BatchLaunch L{};
     L.dbl = dbl;
@@ -436,7 +433,7 @@ static bool launch(DmServer s, Slot &sl) {
       L.seg[k].end = end;
     }
     L.ntiles = end;
-    L.grid = std::min<long long>(end, resident);
+    L.grid = end;  // one workgroup per tile
     ok = dm::launch_ok(nullptr, dm::launch_batch_pass(L, st), "dm_server: batched pass launch");
   }
   ok = ok && hipEventRecord(sl.ev[2], st) == hipSuccess;

@@ -26,6 +26,9 @@ typedef struct dm_batch *DmBatch;
 #define DM_BATCH_TILE_HALF 1    /* tiles of up to half of it */
 #define DM_BATCH_RADIX_LARGE_FIRST 0
 #define DM_BATCH_RADIX_SMALL_FIRST 1
+/* Launch grids (dm_batch_set_grid, dm_batch_autotune). */
+#define DM_BATCH_GRID_TILES 0       /* one workgroup per tile (the default) */
+#define DM_BATCH_GRID_PERSISTENT 1  /* as many workgroups as fit the device at once, each looping over tiles */
 
 /** count transforms of ndim (1..3) axes; shapes[i * ndim + d] is axis d of
  *  transform i.  Axis lengths must factor into 2, 3, 5, 7, 11, 13 with at
@@ -36,13 +39,18 @@ DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int precision, int
 int dm_batch_bind(DmBatch b, void *const *data);
 /** Tile and radix schedule (the defaults: FULL, LARGE_FIRST); rebinds. */
 int dm_batch_set_schedule(DmBatch b, int tile, int radix_order);
+/** Launch grid (DM_BATCH_GRID_*); rebinds. */
+int dm_batch_set_grid(DmBatch b, int mode);
+/** The current launch grid mode. */
+int dm_batch_grid(DmBatch b);
 /** Enqueue the transform on a HIP stream (NULL: the legacy default stream). */
 int dm_batch_run(DmBatch b, void *stream);
 /** First call: capture dm_batch_run into a HIP graph; every call launches
  *  the graph on the stream.  Rebinding or rescheduling drops the graph. */
 int dm_batch_graph_launch(DmBatch b, void *stream);
-/** Times every tile/radix schedule on the bound data (which it overwrites)
- *  and keeps the fastest; returns its milliseconds per run (< 0: error). */
+/** Times every grid, tile and radix-order choice on the bound data (which
+ *  it overwrites) and keeps the fastest; returns its milliseconds per run
+ *  (< 0: error). */
 double dm_batch_autotune(DmBatch b, void *stream);
 /** Tiles per axis pass and launch grid of the current binding (inspection). */
 int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice, int *radix_order);
