@@ -163,15 +163,18 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     L.tiles_u = 1;
   }
   // Non-temporal data accesses measured faster on every pass (512^3: up to
-  // 10%) except a complex-double one whose output is a column walk at a
-  // line stride of 64 KiB or more (the wide tile's 256-byte rows), which
-  // they slow by about as much; complex-float ones gain 2% there too.
+  // 10%) except the stores of a pass whose output is a column walk at a line
+  // stride of 64 KiB or more: there the hint on the loads alone is best
+  // (complex float: 2.58 -> 2.50 ms per 512^3 step against the hint on both).
   {
+    // (complex double: no hint at all there, 4.63 ms against 4.85 with the
+    // hint on its loads)
     const size_t es = dbl ? 16 : 8;
     static int nont = -1;
     if (nont < 0) nont = getenv("DM_NO_NONTEMPORAL") != nullptr;
-    const bool far_out = dbl && !out_row && (size_t)U.h.m[t] * es >= (64u << 10);
-    L.stream_in = L.stream_out = (!far_out && !nont) ? 1 : 0;
+    const bool far = !out_row && (size_t)U.h.m[t] * es >= (64u << 10);
+    L.stream_in = (nont || (far && dbl)) ? 0 : 1;
+    L.stream_out = (nont || far) ? 0 : 1;
   }
   L.nbatch = 0;
   for (int d = 0; d < nd; d++)
