@@ -381,10 +381,11 @@ __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) vo
 
 inline unsigned long long magic_for(int d) { return (0x100000000ULL / (unsigned long long)d) + 1ULL; }
 
-// Big tiles: 8 lines of 512 complex doubles (16 of complex floats) on 512
-// threads; two such workgroups share a CU.  A 1024-thread tile of 16 lines
-// (256-byte column rows) measured slower: one workgroup per CU hides less
-// latency than the wider rows gain.
+// Big tiles: 8 lines of 512 complex doubles on 512 threads (two such
+// workgroups share a CU), 16 lines of 512 complex floats on 1024 threads.
+// For complex double, a 1024-thread tile of 16 lines (256-byte column rows)
+// measured slower except on column walks at a 64 KiB or larger line
+// stride, where it is used (launch_tiles).
 constexpr int CAP_D = 4096;  // complex double points per big tile
 constexpr int CAP_F = 8192;  // complex float points per big tile
 constexpr int NTH_BIG = 512;
@@ -412,10 +413,10 @@ hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     attr_set = true;
   }
-  // Persistent grid: as many workgroups as fit on the device at once.  The
-  // runtime's occupancy query assumes 64 KiB of LDS per CU; gfx950 has 160
-  // KiB, so residency is computed here from LDS and VGPR use (512 VGPRs per
-  // SIMD lane, at most 8 waves per SIMD, 4 SIMDs per CU).
+  // One workgroup per tile.  Residency, reported under DM_DEBUG, is
+  // computed here from LDS and VGPR use (512 VGPRs per SIMD lane, at most 8
+  // waves per SIMD, 4 SIMDs per CU): the runtime's occupancy query assumes
+  // 64 KiB of LDS per CU, and gfx950 has 160 KiB.
   static int vgprs = -1;
   if (vgprs < 0) {
     hipFuncAttributes fa;

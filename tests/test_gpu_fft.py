@@ -145,6 +145,7 @@ def owned_mask(d, width):
     (8, (16, 8, 32), (2, 2, 2), [2, 8, 4]),
     (4, (12, 12, 12), (1, 2, 2), None),           # radix 3
     (6, (12, 20, 14), (1, 2, 3), [3, 4, 7]),      # radices 3, 5, 7
+    (4, (22, 26, 16), (1, 2, 2), [2, 2, 4]),      # radices 11, 13
     (4, (16, 16), (2, 2), None),                  # 2D
     (4, (16, 16, 16), (1, 2, 2), [16, 16, 16]),   # single stage (A = 1)
     (4, (16, 16, 16), (1, 2, 2), [1, 1, 8]),      # B_d = 1 beside B_d > 1
@@ -171,6 +172,7 @@ def test_split_fft_slab(C):
     (8, (32, 16, 32), (2, 2, 2), [4, 4, 8]),
     (8, (16, 16, 16), (2, 2, 2), [1, 4, 1]),
     (8, (16, 16, 16), (1, 2, 4), [1, 1, 16]),
+    (4, (22, 26, 16), (1, 2, 2), [2, 2, 4]),      # radices 11, 13
 ])
 def test_split_fft_fp32_vs_fp64_oracle(C, P, shape, grid, B):
     if B is None:
@@ -185,6 +187,7 @@ PHASED = [
     (4, (16, 16, 16), (1, 2, 2), [4, 4, 4], None),
     (4, (16, 16, 16), (1, 2, 2), [4, 2, 8], None),     # A != B
     (6, (12, 20, 14), (1, 2, 3), [3, 4, 7], None),     # radices 3, 5, 7, uneven blocks
+    (4, (22, 26, 16), (1, 2, 2), [2, 2, 4], None),     # 11- and 13-point second stages
     (4, (16, 16, 16), (1, 2, 2), [16, 16, 16], None),  # single stage (A = 1)
     (8, (16, 16, 16), (2, 2, 2), [1, 4, 1], None),     # B_d = 1 beside B_d > 1
     (2, (8, 8, 8), (1, 1, 2), [1, 1, 1], 4096),        # single second stage (B = 1)

@@ -135,6 +135,34 @@ def ax_pos(lay, r, i):
     return (r // lay.rs[i]) % rm.proc_size(lay.descs[i])
 
 
+@pytest.mark.parametrize("shape,grid", [((8, 6, 10), (2, 3, 1)), ((12, 4, 8), (1, 2, 4)), ((6, 6), (3, 2))])
+def test_even_packed_block_is_c_order(C, shape, grid):
+    """Closed form for makePackedDecomp (reshape.c:203-248) on evenly
+    divided blocks: each rank owns one block of L_d / p_d points per axis,
+    stores it as the C-order array of those local sizes, and the blocks of
+    the P ranks are distinct."""
+    P = int(np.prod(grid))
+    loc = [n // p for n, p in zip(shape, grid)]
+    comm = C.dm_world_create(P)
+    blocks = set()
+    for r in range(P):
+        pd = C.makePackedDecomp(C.dm_comm_view(comm, r), len(shape),
+                                dm.ptrs([prod_dim(C, ("block", n, p)) for n, p in zip(shape, grid)]))
+        assert C.extent(pd) == int(np.prod(loc))
+        mine = []
+        for g in itertools.product(*[range(n) for n in shape]):
+            if not C.areCoordsLocal(pd, dm.ints(list(g))):
+                continue
+            rel = [x % l for x, l in zip(g, loc)]
+            assert C.getOffset(pd, dm.ints(list(g))) == int(np.ravel_multi_index(rel, loc))
+            mine.append(tuple(x // l for x, l in zip(g, loc)))
+        assert len(mine) == int(np.prod(loc)) and len(set(mine)) == 1
+        blocks.add(mine[0])
+        C.freeDecomp(pd)
+    assert len(blocks) == P
+    C.dm_world_free(comm)
+
+
 def test_compatible(C):
     comm = C.dm_world_create(4)
     a = C.makePackedDecomp(comm, 2, dm.ptrs([prod_dim(C, ("block", 8, 2)), prod_dim(C, ("block", 6, 2))]))
