@@ -30,6 +30,7 @@ SZ = ctypes.c_size_t
 FFTW_FORWARD = -1
 FFTW_BACKWARD = 1
 NO_FLAGS, PREFER_LARGE_FFT, PREFER_STAGE1, FORCE_USE_FIRST, RUN_PHASE_ORDER = 0, 1, 2, 4, 0x100
+DM_TWIDDLES_COMPENSATED, DM_TWIDDLES_PLAIN = 0, 1
 RESHAPE_1, FFT_1, TWIDDLE, RESHAPE_2, FFT_2, ALL, NUM_TIMERS = range(7)
 
 
@@ -130,6 +131,7 @@ _SIGS = {
     "makeParallelPlanf": (P, [P, IP, P, I, P]),
     "getTransformed": (P, [P]), "runSplitPlan": (None, [P, I]), "freeSplitPlan": (None, [P]),
     "dm_split_set_phase_order": (I, [P, I]), "dm_split_phase_order": (I, [P]),
+    "dm_split_set_twiddles": (I, [P, I]), "dm_split_twiddles": (I, [P]),
     "dm_split_set_output": (I, [P, P]),
     "makeInterleavedDecomp": (P, [I, P, IP, IP, IP]), "makeBlockedDecomp": (P, [I, P, IP, IP, IP]),
     "gcd": (ctypes.c_uint, [ctypes.c_uint, ctypes.c_uint]), "decomp_scale": (None, [P, P, P, I]),

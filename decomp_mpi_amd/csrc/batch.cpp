@@ -250,12 +250,9 @@ static bool bind_tables(DmBatch b) {
   };
   b->d_ptrs = up(b->data.data(), b->data.size() * sizeof(void *));
   b->d_sched = up(sched.data(), sched.size() * sizeof(dm::BatchSched));
-  if (b->dbl) {
-    b->d_tw = up(tw.data(), tw.size() * sizeof(double));
-  } else {
-    std::vector<float> twf(tw.begin(), tw.end());
-    b->d_tw = up(twf.data(), twf.size() * sizeof(float));
-  }
+  std::vector<char> twimg(tw.size() / 2 * 16);
+  dm::twiddle_image(tw.data(), tw.size() / 2, b->dbl, twimg.data());
+  b->d_tw = up(twimg.data(), twimg.size());
   if (!b->d_ptrs || !b->d_sched || !b->d_tw) {
     raise_error(nullptr, "dm_batch_bind: device allocation or copy failed");
     return false;

@@ -96,7 +96,8 @@ __device__ __forceinline__ void load_first(cx<R> *buf, const BatchSched &S, cons
 // (stage-ordered: Ns = nr in the last stage), transform, store.
 template <typename R, bool INV, int RADIX, int CAP, int NTH>
 __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S, const BatchTile &d,
-                                           gptr<R> data, long long lstride, int ls, const cx<R> *tw) {
+                                           gptr<R> data, long long lstride, int ls,
+                                           const typename tw_global<R>::type *tw) {
   constexpr int K = (CAP / NTH) / RADIX;
   const StageInfo &st = S.st[S.nstage - 1];
   const int nr = st.nr, nb = S.T * nr;
@@ -110,11 +111,7 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 #pragma unroll
     for (int r = 0; r < RADIX; r++) v[r] = buf[l * ls + lds_pad(j + r * nr)];
 #pragma unroll
-    for (int r = 1; r < RADIX; r++) {
-      cx<R> w = tw[(nr - 1) + (r - 1) * nr + j];
-      if (INV) w.y = -w.y;
-      v[r] = cmul(v[r], w);
-    }
+    for (int r = 1; r < RADIX; r++) v[r] = twmul<INV, (sizeof(R) == 4)>(v[r], tw, (nr - 1) + (r - 1) * nr + j);
     dft<R, INV, RADIX>(v);
     if (ok)
 #pragma unroll
@@ -140,7 +137,8 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     // or from the segment (one transform's own tables) that holds tile t
     BatchTile d;
     const BatchSched *sched;
-    const cx<R> *twall;
+    typedef typename tw_global<R>::type TwE;
+    const TwE *twall;
     void *xdata;
     if (p.nseg > 0) {
       int sg = 0;
@@ -164,12 +162,12 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
       d.xform = 0;
       d.sched = 0;
       sched = g.sched;
-      twall = reinterpret_cast<const cx<R> *>(g.twiddle);
+      twall = reinterpret_cast<const TwE *>(g.twiddle);
       xdata = g.data;
     } else {
       d = p.tiles[t];
       sched = p.sched;
-      twall = reinterpret_cast<const cx<R> *>(p.twiddle);
+      twall = reinterpret_cast<const TwE *>(p.twiddle);
       xdata = p.data[d.xform];
     }
     const BatchSched &S = sched[d.sched];
@@ -180,7 +178,7 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     // the transform's pointer comes from memory, so its address space is
     // stated: through a generic pointer every access would be a flat one
     gptr<R> data = (gptr<R>)(xdata) + d.base;
-    const cx<R> *tw = twall + S.tw_off;
+    const TwE *tw = twall + S.tw_off;
     // Fused first / last stages: a power-of-two radix dividing the points per
     // thread (whole butterflies per thread); the last one needs two stages.
     const int r0 = S.radix[0], rl = S.radix[S.nstage - 1];
@@ -226,15 +224,15 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     const int s0 = fuse_in ? 1 : 0, s1 = fuse_out ? S.nstage - 1 : S.nstage;
     for (int s = s0; s < s1; s++) {
       switch (S.radix[s]) {
-        case 2: stage_lds<R, INV, 2, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 3: stage_lds<R, INV, 3, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 4: stage_lds<R, INV, 4, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 5: if constexpr (PRIMES) stage_lds<R, INV, 5, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 7: if constexpr (PRIMES) stage_lds<R, INV, 7, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 8: stage_lds<R, INV, 8, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
-        case 16: stage_lds<R, INV, 16, CAP, NTH>(buf, S.st[s], T, ls, tw); break;
+        case 2: stage_lds<R, INV, 2, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 3: stage_lds<R, INV, 3, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 4: stage_lds<R, INV, 4, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 5: if constexpr (PRIMES) stage_lds<R, INV, 5, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 7: if constexpr (PRIMES) stage_lds<R, INV, 7, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 8: stage_lds<R, INV, 8, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
+        case 16: stage_lds<R, INV, 16, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
         default: break;
       }
     }

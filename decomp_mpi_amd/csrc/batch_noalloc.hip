@@ -103,7 +103,7 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
   const size_t o_ptr = 0;
   const size_t o_sched = align256((size_t)count * sizeof(void *));
   const size_t o_tw = o_sched + align256(h.sched.size() * sizeof(dm::BatchSched));
-  const size_t tw_bytes = (h.tw.size() / 2) * (dbl ? 16 : 8);
+  const size_t tw_bytes = (h.tw.size() / 2) * 16;
   std::vector<size_t> o_tiles(ndim);
   size_t at = o_tw + align256(tw_bytes);
   for (int p = 0; p < ndim; p++) {
@@ -121,12 +121,7 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
   std::vector<char> img(at, 0);
   memcpy(img.data() + o_ptr, data, (size_t)count * sizeof(void *));
   memcpy(img.data() + o_sched, h.sched.data(), h.sched.size() * sizeof(dm::BatchSched));
-  if (dbl) {
-    memcpy(img.data() + o_tw, h.tw.data(), tw_bytes);
-  } else {
-    float *f = (float *)(img.data() + o_tw);
-    for (size_t i = 0; i < h.tw.size(); i++) f[i] = (float)h.tw[i];
-  }
+  dm::twiddle_image(h.tw.data(), h.tw.size() / 2, dbl, img.data() + o_tw);
   for (int p = 0; p < ndim; p++)
     memcpy(img.data() + o_tiles[p], h.tiles[p].data(), h.tiles[p].size() * sizeof(dm::BatchTile));
   hipStream_t s = (hipStream_t)stream;

@@ -21,6 +21,64 @@ template <typename R>
 __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
+// Twiddle table entries in device memory, 16 bytes per root.  Complex
+// double: the root itself.  Complex float: a compensated pair, the root w
+// (formed in long double, rounded to double) as hi = fl32(w) and lo =
+// w - hi rounded to bfloat16 (both parts of lo packed in one word, real
+// part in the low half), so the entry holds w to about 2^-33 where a float
+// alone holds it to 2^-25.  (hi.re, hi.im, lo) is one vector: a single
+// 12-byte load.
+typedef float twf __attribute__((ext_vector_type(3)));
+template <typename R>
+struct tw_global {
+  typedef cx<R> type;
+};
+template <>
+struct tw_global<float> {
+  typedef twf type;
+};
+// Entry of a table copied to LDS: the whole entry for compensated products
+// (CT), the root's (re, im) otherwise.
+template <typename R, bool CT>
+struct tw_local {
+  typedef cx<R> type;
+};
+template <>
+struct tw_local<float, true> {
+  typedef twf type;
+};
+
+// The float parts (hi) of a compensated table in device memory, read as
+// 8-byte complex floats at a 16-byte stride, for products without the
+// compensation.
+struct tw_hi {
+  const cx<float> *p;
+  __device__ __forceinline__ cx<float> operator[](int i) const { return p[2 * i]; }
+  __device__ __forceinline__ tw_hi operator+(int k) const { return {p + 2 * k}; }
+};
+
+// a * w for table entry tw[i] (the conjugate root for the inverse).  CT
+// (complex float only): compensated product, a * lo formed first and added
+// inside the fused multiply-adds of a * hi, so that the result carries the
+// rounding of the float arithmetic but not the rounding of w to float.
+// Without CT only the entry's (re, im) is read.
+template <bool INV, bool CT, typename R, typename TW>
+__device__ __forceinline__ cx<R> twmul(cx<R> a, TW tw, int i) {
+  if constexpr (CT) {
+    static_assert(sizeof(R) == 4, "compensated twiddles are for complex float");
+    const twf e = tw[i];
+    const unsigned lo = __float_as_uint(e.z);
+    const float lx = __uint_as_float(lo << 16), ly0 = __uint_as_float(lo & 0xffff0000u);
+    const float hx = e.x, hy = INV ? -e.y : e.y, ly = INV ? -ly0 : ly0;
+    const float cr = a.x * lx - a.y * ly;
+    const float ci = a.x * ly + a.y * lx;
+    return {__builtin_fmaf(a.x, hx, __builtin_fmaf(-a.y, hy, cr)), __builtin_fmaf(a.x, hy, __builtin_fmaf(a.y, hx, ci))};
+  } else {
+    const R wx = tw[i].x, wy = INV ? -tw[i].y : tw[i].y;
+    return {a.x * wx - a.y * wy, a.x * wy + a.y * wx};
+  }
+}
+
 // multiply by -i (forward) or +i (inverse)
 template <typename R, bool INV>
 __device__ __forceinline__ cx<R> rot(cx<R> a) {
@@ -201,8 +259,9 @@ __device__ __forceinline__ int lds_index(int line, int x, int ls) { return line 
 // butterflies read points j + r * nr of their line, get the stage twiddles
 // (stage-ordered table), transform, and write back to their Stockham places.
 // TW: pointer to the twiddles, typed by where they live (an LDS-qualified
-// pointer reads with LDS instructions; a generic one would use flat loads).
-template <typename R, bool INV, int RADIX, int CAP, int NTH, typename TW>
+// pointer reads with LDS instructions; a generic one would use flat loads);
+// CT: compensated twiddle products (complex float).
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool CT, typename TW>
 __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T, int ls, TW tw) {
   constexpr int K = (CAP / RADIX + NTH - 1) / NTH;
   const int nr = si.nr, ns = si.ns;
@@ -228,11 +287,7 @@ __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T
         // stage-ordered table: consecutive jj read consecutive entries
         TW tws = tw + (ns - 1) + jj;
 #pragma unroll
-        for (int r = 1; r < RADIX; r++) {
-          cx<R> w = {tws[(r - 1) * ns].x, tws[(r - 1) * ns].y};
-          if (INV) w.y = -w.y;
-          v[k][r] = cmul(v[k][r], w);
-        }
+        for (int r = 1; r < RADIX; r++) v[k][r] = twmul<INV, CT>(v[k][r], tws, (r - 1) * ns);
       }
       dft<R, INV, RADIX>(v[k]);
       // line and Stockham-expanded index of the first output

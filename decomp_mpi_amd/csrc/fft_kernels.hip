@@ -31,6 +31,15 @@ int fft_tile_capacity(int dbl, int column, int wide) {
   return small ? cap / 2 : cap;
 }
 
+// Twiddle table in LDS: 16 bytes per entry for a complex double or a
+// compensated complex-float pair, 8 for a plain complex float.  A
+// compensated table stays within 8 KiB there, so that two 8192-point
+// complex-float tiles still share a CU.
+static size_t twiddle_lds_entry(const FftLaunch &p) { return (p.dbl || p.ctw) ? 16 : 8; }
+static bool twiddle_in_lds(const FftLaunch &p) {
+  return (size_t)p.N * twiddle_lds_entry(p) <= ((!p.dbl && p.ctw) ? 8192u : 16384u);
+}
+
 size_t fft_lds_bytes(const FftLaunch &p) {
   using namespace fftpass;
   size_t elem = p.dbl ? 16 : 8;
@@ -39,7 +48,7 @@ size_t fft_lds_bytes(const FftLaunch &p) {
   // tile data + per-line address data of both sides
   size_t b = (size_t)p.T * ls * elem + side(p.sin.uniform) + side(p.sout.uniform);
   b = (b + 15) & ~(size_t)15;
-  if ((size_t)p.N * elem <= 16384) b += (size_t)p.N * elem;  // twiddle table
+  if (twiddle_in_lds(p)) b += (size_t)p.N * twiddle_lds_entry(p);
   return b;
 }
 
@@ -54,7 +63,7 @@ int launch_fft_pass(const FftLaunch &p, void *stream) {
     // Fused first/last stages need whole butterflies per thread (points per
     // thread a multiple of the radix) and an unpermuted walk along the line
     // in row mode (a permuted one goes through LDS to stay coalesced).
-    const int V = p.dbl ? CAP_D / NTH_BIG : CAP_F / f32_threads();
+    const int V = 8;  // points per thread in every tile
     auto pow2 = [](int r) { return r == 2 || r == 4 || r == 8 || r == 16; };
     int r0 = p.radix[0], rl = p.radix[p.nstage > 0 ? p.nstage - 1 : 0];
     P.fuse_in = p.nstage >= 1 && pow2(r0) && V % r0 == 0 && (!p.in_row || p.ord_in == nullptr);
@@ -62,13 +71,17 @@ int launch_fft_pass(const FftLaunch &p, void *stream) {
     static int off = -1;
     if (off < 0) off = getenv("DM_NO_FUSE") != nullptr;
     if (off) P.fuse_in = P.fuse_out = 0;
+    // compensated twiddles with per-rank output strides: the fused store's
+    // address arithmetic and the twiddle pairs do not fit 64 VGPRs, so the
+    // last stage runs in LDS (the kernel has no fused store for this form)
+    if (!p.dbl && p.ctw && !p.sout.uniform) P.fuse_out = 0;
     // column walks with rows shorter than 128 bytes: pair the tiles that
     // share lines on one XCD
     int elem = p.dbl ? 16 : 8;
     int row_bytes = p.T * elem;
     bool col = !p.in_row || !p.out_row;
     P.xcd_pair = (col && p.u >= 0 && row_bytes < 128) ? 128 / row_bytes : 1;
-    P.tw_lds = (size_t)p.N * elem <= 16384;
+    P.tw_lds = twiddle_in_lds(p);
   }
   int ns = 1;
   for (int s = 0; s < p.nstage; s++) {

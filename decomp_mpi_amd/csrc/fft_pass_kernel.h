@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 #include "fft_device.h"
 #include "kernels.h"
@@ -36,9 +37,9 @@ struct Pass {
   int tw_lds;                    // twiddle table copied to LDS
 };
 
-template <typename R, bool INV, int RADIX, int CAP, int NTH, typename TW>
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool CT, typename TW>
 __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, TW tw) {
-  stage_lds<R, INV, RADIX, CAP, NTH>(buf, P.st[s], P.p.T, P.ls, tw);
+  stage_lds<R, INV, RADIX, CAP, NTH, CT>(buf, P.st[s], P.p.T, P.ls, tw);
 }
 
 // Per-line address data of one side of a tile (in LDS).
@@ -52,6 +53,8 @@ typedef __attribute__((address_space(3))) i64 lds_i64;
 typedef __attribute__((address_space(3))) int lds_int;
 template <typename R>
 using lds_cx = __attribute__((address_space(3))) cx<R>;
+template <typename R, bool CT>
+using lds_tw = __attribute__((address_space(3))) typename tw_local<R, CT>::type;
 
 template <bool UNIFORM>
 struct LineAddr;
@@ -70,6 +73,12 @@ struct LineAddr<true> {
   __device__ __forceinline__ i64 at(const FftLaunch &, const FftSide &s, const i64 *absT, int l, int x) const {
     if (s.aff) return base[l] + (i64)x * s.aff;  // affine along t: no table load
     return base[l] + absT[x];
+  }
+  // the same with the side's form fixed at compile time (no branch)
+  template <bool AFF>
+  __device__ __forceinline__ i64 at_form(const FftSide &s, const i64 *absT, int l, int x) const {
+    if constexpr (AFF) return base[l] + (i64)x * s.aff;
+    else return base[l] + absT[x];
   }
 };
 
@@ -210,7 +219,11 @@ __device__ __forceinline__ void load_first_stage(cx<R> *buf, const Pass &P, cons
 
 // Last stage straight to memory: read butterfly inputs from LDS, twiddle,
 // transform, and store outputs j + nr*r of line l.
-template <typename R, bool INV, int RADIX, int CAP, int NTH, bool UOUT, typename TW>
+// AFF (uniform output sides, compensated twiddles): the output is affine
+// along the line, chosen once per launch, so that no branches sit between
+// the twiddle products and the stores (with them the compensated form
+// spills at 64 VGPRs).
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool UOUT, bool CT, bool AFF, typename TW>
 __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P, const LineAddr<UOUT> &lout,
                                                  cx<R> *out, const i64 *outT, TW tw, int nl) {
   constexpr int K = (CAP / NTH) / RADIX;
@@ -232,16 +245,14 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
     for (int r = 0; r < RADIX; r++) v[q][r] = buf[base + lds_pad(j + r * nr)];
     // in the last stage ns = nr, so j % ns = j (stage-ordered table)
 #pragma unroll
-    for (int r = 1; r < RADIX; r++) {
-      const int ti = (nr - 1) + (r - 1) * nr + j;
-      cx<R> w = {tw[ti].x, tw[ti].y};
-      if (INV) w.y = -w.y;
-      v[q][r] = cmul(v[q][r], w);
-    }
+    for (int r = 1; r < RADIX; r++) v[q][r] = twmul<INV, CT>(v[q][r], tw, (nr - 1) + (r - 1) * nr + j);
     dft<R, INV, RADIX>(v[q]);
     int ls = ok[q] ? l : 0;
 #pragma unroll
-    for (int r = 0; r < RADIX; r++) a[q][r] = lout.at(p, p.sout, outT, ls, j + r * nr);
+    for (int r = 0; r < RADIX; r++) {
+      if constexpr (UOUT && CT) a[q][r] = lout.template at_form<AFF>(p.sout, outT, ls, j + r * nr);
+      else a[q][r] = lout.at(p, p.sout, outT, ls, j + r * nr);
+    }
   }
   scatter<K * RADIX, RADIX>(out, &a[0][0], &v[0][0], ok, p.stream_out);
 }
@@ -253,7 +264,8 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
 // latency part of every later address computation.
 // Complex float at 8 points per thread is held to 64 VGPRs (8 waves per
 // SIMD: two 1024-thread or four 512-thread workgroups per CU).
-template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT>
+// CT: compensated twiddle products (complex float only).
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT>
 __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) void fft_pass_kernel(Pass P) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
@@ -284,11 +296,15 @@ __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) vo
   // twiddle table in LDS after the line data
   size_t tw_at = (size_t)T * P.ls * sizeof(cx<R>) + LineAddr<UIN>::bytes(T, p.ndim) +
                  LineAddr<UOUT>::bytes(T, p.ndim);
-  cx<R> *twl = reinterpret_cast<cx<R> *>(smem_raw + ((tw_at + 15) & ~(size_t)15));
-  if (P.tw_lds) {
-    const cx<R> *twg = reinterpret_cast<const cx<R> *>(p.twiddle);
-    for (int i = threadIdx.x; i < p.N; i += NTH) twl[i] = twg[i];
-  }
+  typedef typename tw_local<R, CT>::type TwL;
+  typedef typename tw_global<R>::type TwG;
+  TwL *twl = reinterpret_cast<TwL *>(smem_raw + ((tw_at + 15) & ~(size_t)15));
+  const TwG *twg = reinterpret_cast<const TwG *>(p.twiddle);
+  if (P.tw_lds)
+    for (int i = threadIdx.x; i < p.N; i += NTH) {
+      if constexpr (sizeof(TwL) == sizeof(TwG)) twl[i] = twg[i];
+      else twl[i] = TwL{twg[i].x, twg[i].y};
+    }
   __syncthreads();
 
   const i64 *inT = p.off_in[p.t];
@@ -333,31 +349,42 @@ __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) vo
   auto run_stages = [&](auto tw) {
     for (int s = s0; s < s1; s++) {
       switch (p.radix[s]) {
-        case 2: stage<R, INV, 2, CAP, NTH>(buf, P, s, tw); break;
-        case 3: if constexpr (!POW2) stage<R, INV, 3, CAP, NTH>(buf, P, s, tw); break;
-        case 4: stage<R, INV, 4, CAP, NTH>(buf, P, s, tw); break;
-        case 5: if constexpr (!POW2) stage<R, INV, 5, CAP, NTH>(buf, P, s, tw); break;
-        case 7: if constexpr (!POW2) stage<R, INV, 7, CAP, NTH>(buf, P, s, tw); break;
-        case 8: stage<R, INV, 8, CAP, NTH>(buf, P, s, tw); break;
-        case 11: if constexpr (!POW2) stage<R, INV, 11, CAP, NTH>(buf, P, s, tw); break;
-        case 13: if constexpr (!POW2) stage<R, INV, 13, CAP, NTH>(buf, P, s, tw); break;
-        case 16: stage<R, INV, 16, CAP, NTH>(buf, P, s, tw); break;
+        case 2: stage<R, INV, 2, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 3: if constexpr (!POW2) stage<R, INV, 3, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 4: stage<R, INV, 4, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 5: if constexpr (!POW2) stage<R, INV, 5, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 7: if constexpr (!POW2) stage<R, INV, 7, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 8: stage<R, INV, 8, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 11: if constexpr (!POW2) stage<R, INV, 11, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 13: if constexpr (!POW2) stage<R, INV, 13, CAP, NTH, CT>(buf, P, s, tw); break;
+        case 16: stage<R, INV, 16, CAP, NTH, CT>(buf, P, s, tw); break;
         default: break;
       }
     }
 
     if (P.fuse_out) {
-      switch (p.radix[p.nstage - 1]) {
-        case 2: if constexpr ((CAP / NTH) % 2 == 0) last_stage_store<R, INV, 2, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-        case 4: if constexpr ((CAP / NTH) % 4 == 0) last_stage_store<R, INV, 4, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-        case 8: if constexpr ((CAP / NTH) % 8 == 0) last_stage_store<R, INV, 8, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-        case 16: if constexpr ((CAP / NTH) % 16 == 0) last_stage_store<R, INV, 16, CAP, NTH, UOUT>(buf, P, lout, out, outT, tw, nl); break;
-        default: break;
+      auto store = [&](auto aff) {
+        constexpr bool AFF = decltype(aff)::value;
+        switch (p.radix[p.nstage - 1]) {
+          case 2: if constexpr ((CAP / NTH) % 2 == 0) last_stage_store<R, INV, 2, CAP, NTH, UOUT, CT, AFF>(buf, P, lout, out, outT, tw, nl); break;
+          case 4: if constexpr ((CAP / NTH) % 4 == 0) last_stage_store<R, INV, 4, CAP, NTH, UOUT, CT, AFF>(buf, P, lout, out, outT, tw, nl); break;
+          case 8: if constexpr ((CAP / NTH) % 8 == 0) last_stage_store<R, INV, 8, CAP, NTH, UOUT, CT, AFF>(buf, P, lout, out, outT, tw, nl); break;
+          case 16: if constexpr ((CAP / NTH) % 16 == 0) last_stage_store<R, INV, 16, CAP, NTH, UOUT, CT, AFF>(buf, P, lout, out, outT, tw, nl); break;
+          default: break;
+        }
+      };
+      if constexpr (CT && !UOUT) {
+        // not built: the host turns the fused store off for this form
+      } else if (CT && p.sout.aff) {
+        store(std::true_type{});
+      } else {
+        store(std::false_type{});
       }
     }
   };
-  if (P.tw_lds) run_stages((const lds_cx<R> *)twl);
-  else run_stages(reinterpret_cast<const cx<R> *>(p.twiddle));
+  if (P.tw_lds) run_stages((const lds_tw<R, CT> *)twl);
+  else if constexpr (sizeof(R) == 4 && !CT) run_stages(tw_hi{reinterpret_cast<const cx<R> *>(twg)});
+  else run_stages(twg);
   if (!P.fuse_out) {
     i64 a[V];
     int sl[V];
@@ -389,22 +416,11 @@ inline unsigned long long magic_for(int d) { return (0x100000000ULL / (unsigned 
 constexpr int CAP_D = 4096;  // complex double points per big tile
 constexpr int CAP_F = 8192;  // complex float points per big tile
 constexpr int NTH_BIG = 512;
-// Threads of a big complex-float tile: 1024 (8 points per thread, as for
-// complex double) unless DM_F32_THREADS=512 (16 points per thread).
-inline int f32_threads() {
-  static int n = -1;
-  if (n < 0) {
-    const char *e = getenv("DM_F32_THREADS");
-    n = (e != nullptr && atoi(e) == 512) ? 512 : 1024;
-  }
-  return n;
-}
-
-template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT>
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT>
 hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
   static bool attr_set = false;
   static int cus = 0;
-  auto k = fft_pass_kernel<R, INV, CAP, NTH, POW2, UIN, UOUT>;
+  auto k = fft_pass_kernel<R, INV, CAP, NTH, POW2, UIN, UOUT, CT>;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
@@ -437,40 +453,44 @@ hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename R, bool INV, int CAP, int NTH, bool POW2>
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool CT>
 hipError_t launch_layouts(const Pass &P, size_t lds, hipStream_t s) {
   bool ui = P.p.sin.uniform, uo = P.p.sout.uniform;
-  if (ui && uo) return launch_typed<R, INV, CAP, NTH, POW2, true, true>(P, lds, s);
-  if (ui) return launch_typed<R, INV, CAP, NTH, POW2, true, false>(P, lds, s);
-  if (uo) return launch_typed<R, INV, CAP, NTH, POW2, false, true>(P, lds, s);
-  return launch_typed<R, INV, CAP, NTH, POW2, false, false>(P, lds, s);
+  if (ui && uo) return launch_typed<R, INV, CAP, NTH, POW2, true, true, CT>(P, lds, s);
+  if (ui) return launch_typed<R, INV, CAP, NTH, POW2, true, false, CT>(P, lds, s);
+  if (uo) return launch_typed<R, INV, CAP, NTH, POW2, false, true, CT>(P, lds, s);
+  return launch_typed<R, INV, CAP, NTH, POW2, false, false, CT>(P, lds, s);
+}
+
+template <typename R, bool INV, int CAP, int NTH, bool POW2>
+hipError_t launch_twiddles(const Pass &P, size_t lds, hipStream_t s) {
+  if constexpr (sizeof(R) == 4)
+    if (P.p.ctw) return launch_layouts<R, INV, CAP, NTH, POW2, true>(P, lds, s);
+  return launch_layouts<R, INV, CAP, NTH, POW2, false>(P, lds, s);
 }
 
 // Power-of-two radix schedules get a kernel without the prime butterflies
-// (fewer registers); the thread count keeps 8 (FP64) or 16 (FP32) points
-// per thread in the load and store phases.
-// Two tile sizes with the same points per thread: CAP points on 512
-// threads, or CAP/2 points on 256 threads (half the LDS, twice the
-// workgroups per CU).
+// (fewer registers).  Every tile holds 8 points per thread in the load and
+// store phases.  Complex double: CAP points on 512 threads, or CAP/2 on 256
+// (half the LDS, twice the workgroups per CU), or the wide tile.  Complex
+// float: CAP points on 1024 threads or CAP/2 on 512 (at 16 points per
+// thread, 512 threads for CAP measured slower).
 template <typename R, bool INV, bool POW2>
 hipError_t launch_tiles(const Pass &P, size_t lds, hipStream_t s) {
   constexpr int CAP = sizeof(R) == 8 ? CAP_D : CAP_F;
   const bool small = P.p.T * P.p.N <= CAP / 2;
   if constexpr (sizeof(R) == 4) {
-    if (f32_threads() == 1024) {
-      if (small) return launch_layouts<R, INV, CAP / 2, NTH_BIG, POW2>(P, lds, s);
-      return launch_layouts<R, INV, CAP, 2 * NTH_BIG, POW2>(P, lds, s);
-    }
-  }
-  if constexpr (sizeof(R) == 8) {
+    if (small) return launch_twiddles<R, INV, CAP / 2, NTH_BIG, POW2>(P, lds, s);
+    return launch_twiddles<R, INV, CAP, 2 * NTH_BIG, POW2>(P, lds, s);
+  } else {
     // wide complex-double tile: 16 lines (256-byte column rows) on 1024
     // threads, for column walks at a large line stride (for complex float
     // the same 256-byte rows measured slower: 2.66 vs 2.58 ms per 512^3
     // step)
-    if (P.p.T * P.p.N > CAP) return launch_layouts<R, INV, 2 * CAP, 2 * NTH_BIG, POW2>(P, lds, s);
+    if (P.p.T * P.p.N > CAP) return launch_twiddles<R, INV, 2 * CAP, 2 * NTH_BIG, POW2>(P, lds, s);
+    if (small) return launch_twiddles<R, INV, CAP / 2, NTH_BIG / 2, POW2>(P, lds, s);
+    return launch_twiddles<R, INV, CAP, NTH_BIG, POW2>(P, lds, s);
   }
-  if (small) return launch_layouts<R, INV, CAP / 2, NTH_BIG / 2, POW2>(P, lds, s);
-  return launch_layouts<R, INV, CAP, NTH_BIG, POW2>(P, lds, s);
 }
 
 // One translation unit per (precision, direction, radix family) holds the

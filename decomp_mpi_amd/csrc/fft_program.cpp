@@ -83,6 +83,30 @@ std::vector<double> roots(int n) {
   return w;
 }
 
+void twiddle_image(const double *w, size_t n, int dbl, void *dst) {
+  if (dbl) {
+    memcpy(dst, w, n * 2 * sizeof(double));
+    return;
+  }
+  // bfloat16 of x: the high half of its float bits, rounded to nearest even
+  auto bf16 = [](double x) -> uint32_t {
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+  };
+  uint32_t *e = (uint32_t *)dst;
+  for (size_t i = 0; i < n; i++) {
+    const double re = w[2 * i], im = w[2 * i + 1];
+    const float hr = (float)re, hi = (float)im;
+    memcpy(&e[4 * i], &hr, 4);
+    memcpy(&e[4 * i + 1], &hi, 4);
+    e[4 * i + 2] = bf16(re - (double)hr) | (bf16(im - (double)hi) << 16);
+    e[4 * i + 3] = 0;
+  }
+}
+
 bool axis_supported(int n, int dbl, std::string *why) {
   char msg[160];
   std::vector<int> rad;
@@ -113,7 +137,7 @@ static bool is_identity(const std::vector<int> &o) {
 }
 
 bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, bool inverse, const void *in,
-               void *out, TableBuilder &tb, PendingPass *pp) {
+               void *out, TableBuilder &tb, PendingPass *pp, int ctw) {
   int nd = I.h.ndim;
   FftLaunch &L = pp->L;
   memset(&L, 0, sizeof L);
@@ -187,6 +211,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   }
   L.inverse = inverse ? 1 : 0;
   L.dbl = dbl;
+  L.ctw = (!dbl && ctw) ? 1 : 0;
   for (int d = 0; d < nd; d++) {
     pp->in_off[d] = tb.put(I.abs[d].data(), I.abs[d].size() * sizeof(i64));
     pp->out_off[d] = tb.put(U.abs[d].data(), U.abs[d].size() * sizeof(i64));
@@ -282,17 +307,16 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
       ns *= R;
     }
   }
-  if (dbl) {
-    pp->tw = (long)tb.put(w.data(), w.size() * sizeof(double));
-  } else {
-    std::vector<float> wf(w.begin(), w.end());
-    pp->tw = (long)tb.put(wf.data(), wf.size() * sizeof(float));
-  }
+  std::vector<char> img((size_t)L.N * 16);
+  twiddle_image(w.data(), (size_t)L.N, dbl, img.data());
+  pp->tw = (long)tb.put(img.data(), img.size());
   L.in = in;
   L.out = out;
-  logger(comm, "FFT pass: axis %d (N %d, %d stages) lines along %d, T %d, %lld workgroups, in %s stride %lld%s, out %s stride %lld%s%s\n",
+  auto form = [](const FftSide &f) { return std::string(f.aff ? " affine" : ""); };
+  logger(comm, "FFT pass: axis %d (N %d, %d stages) lines along %d, T %d, %lld workgroups, in %s stride %lld%s, out %s stride %lld%s%s%s\n",
          t, L.N, L.nstage, u, L.T, (long long)L.tiles_u * batch, in_row ? "row" : "column", (long long)I.h.m[t],
-         L.sin.aff ? " affine" : "", out_row ? "row" : "column", (long long)U.h.m[t], L.sout.aff ? " affine" : "", L.stream_in ? ", non-temporal" : "");
+         form(L.sin).c_str(), out_row ? "row" : "column", (long long)U.h.m[t], form(L.sout).c_str(),
+         L.stream_in ? ", non-temporal loads" : "", L.stream_out ? ", non-temporal stores" : "");
   return true;
 }
 

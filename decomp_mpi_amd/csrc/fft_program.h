@@ -24,6 +24,12 @@ bool choose_radices(int n, std::vector<int> *out);
 bool axis_supported(int n, int dbl, std::string *why);
 // exp(-2 pi i k / n), k < n, interleaved (re, im), accurate to double rounding
 std::vector<double> roots(int n);
+// Device image of a twiddle table of n roots w (interleaved (re, im)
+// doubles) into dst, 16 bytes per root in either precision: complex double
+// stores the root; complex float stores the compensated pair hi = fl32(w)
+// (two floats) and lo = bf16(w - hi) (real part in the low half of the
+// third word; the fourth word is 0).
+void twiddle_image(const double *w, size_t n, int dbl, void *dst);
 
 // Host image of the device tables of a program.
 struct TableBuilder {
@@ -48,9 +54,10 @@ struct PendingPass {
 };
 
 // Plan the pass over axis t from layout I (buffer in) to layout U (buffer
-// out).  Returns false after raise_error when the axis cannot be done.
+// out); ctw: complex float with compensated twiddle products.  Returns
+// false after raise_error when the axis cannot be done.
 bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, bool inverse, const void *in,
-               void *out, TableBuilder &tb, PendingPass *pp);
+               void *out, TableBuilder &tb, PendingPass *pp, int ctw = 1);
 // Upload the tables and resolve table pointers; returns the device block.
 void *finalize_passes(dm_comm_t comm, TableBuilder &tb, std::vector<PendingPass> &pp,
                       std::vector<FftLaunch> *out);

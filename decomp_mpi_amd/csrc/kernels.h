@@ -79,7 +79,7 @@ struct FftLaunch {
   const int *ord_u;           // tile line l holds u coordinate ord_u[u0 + l] (or u0 + l)
   const i64 *line_in_tab;     // uniform sides: base offset of line l of tile b at [b * T + l]
   const i64 *line_out_tab;    //   (null: computed from the per-axis tables)
-  const void *twiddle;        // stage-ordered roots (N entries, pass precision): stage with
+  const void *twiddle;        // stage-ordered roots (N 16-byte entries, twiddle_image): stage with
                               //   Ns earlier points and radix R holds w^(jj*r*N/(Ns*R)) at
                               //   (Ns-1) + (r-1)*Ns + jj
   int inverse;                // 0 forward, 1 backward (conjugate roots)
@@ -89,6 +89,7 @@ struct FftLaunch {
   long long tile_base;        // this launch runs tiles [tile_base, tile_base + nblocks)
   int pass_index;             // host bookkeeping: the pass this launch belongs to
   int stream_in, stream_out;  // 1: non-temporal hint on the data loads / stores
+  int ctw;                    // complex float: 1 compensated twiddle products, 0 the float part only
 };
 // nblocks == 0: the launch covers every tile of the pass.
 int launch_fft_pass(const FftLaunch &p, void *stream);

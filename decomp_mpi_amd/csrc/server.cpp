@@ -196,16 +196,11 @@ static Sched *sched_for(DmServer s, int N, int T, int dbl) {
   dm::BatchSched one{};
   std::vector<double> tw;
   dm::make_batch_sched(N, T, DM_BATCH_RADIX_LARGE_FIRST, &one, &tw);
-  const size_t tw_bytes = (size_t)N * (dbl ? 16 : 8);
+  const size_t tw_bytes = (size_t)N * 16;
   const size_t o_tw = align256(sizeof one);
   std::vector<char> img(o_tw + tw_bytes, 0);
   memcpy(img.data(), &one, sizeof one);
-  if (dbl) {
-    memcpy(img.data() + o_tw, tw.data(), tw_bytes);
-  } else {
-    float *f = (float *)(img.data() + o_tw);
-    for (size_t i = 0; i < 2 * (size_t)N; i++) f[i] = (float)tw[i];
-  }
+  dm::twiddle_image(tw.data(), (size_t)N, dbl, img.data() + o_tw);
   Sched *p = new Sched;
   p->bytes = img.size();
   p->dev = dm::device_alloc(p->bytes);

@@ -213,6 +213,7 @@ struct dm_split_ext {
   bool phased = false;      // run in the reference's phase order
   bool fused_ok = true;     // every axis fits one pass (else phased only)
   bool custom_out = false;  // out is the caller's (dm_split_set_output)
+  int ctw = 1;              // complex float: compensated twiddle products
   dm_phased_program ph;
 };
 
@@ -408,7 +409,7 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
       si = &O;
       so = (i == n - 1) ? &S : &O;
     }
-    if (!dm::plan_pass(plan->comm, *si, *so, order[i], x->dbl, inverse, bin, bout, tb, &pp[i])) return false;
+    if (!dm::plan_pass(plan->comm, *si, *so, order[i], x->dbl, inverse, bin, bout, tb, &pp[i], x->ctw)) return false;
     prev = bout;
   }
   prog->mem = dm::finalize_passes(plan->comm, tb, pp, &prog->passes);
@@ -515,7 +516,7 @@ static bool compile_stage(SplitPlan plan, const dm::HostLayout &h, std::vector<i
     }
     dm::Side side = dm::make_side(sh, slot);
     const int sub = stage1 ? 2 * d + 1 : 2 * d;
-    if (!dm::plan_pass(plan->comm, side, side, sub, x->dbl, inverse, buf, buf, tb, &pp[i])) return false;
+    if (!dm::plan_pass(plan->comm, side, side, sub, x->dbl, inverse, buf, buf, tb, &pp[i], x->ctw)) return false;
   }
   prog->mem = dm::finalize_passes(plan->comm, tb, pp, &prog->passes);
   return prog->mem != nullptr;
@@ -1187,6 +1188,30 @@ extern "C" int dm_split_set_phase_order(SplitPlan plan, int on) {
   }
   plan->ext->phased = on != 0;
   return DM_SUCCESS;
+}
+
+extern "C" int dm_split_set_twiddles(SplitPlan plan, int mode) {
+  if (plan == nullptr || plan->ext == nullptr) return DM_ERR_ARG;
+  if (mode != DM_TWIDDLES_COMPENSATED && mode != DM_TWIDDLES_PLAIN) {
+    raise_error(plan->comm, "dm_split_set_twiddles: unknown mode %d", mode);
+    return DM_ERR_ARG;
+  }
+  dm_split_ext *x = plan->ext;
+  const int ctw = mode == DM_TWIDDLES_COMPENSATED ? 1 : 0;
+  if (ctw != x->ctw) {
+    // the pass descriptors carry the form: compile again on the next run
+    x->ctw = ctw;
+    program_release(x->fwd);
+    program_release(x->bwd);
+    x->compiled = false;
+    phased_release(x->ph);
+  }
+  return DM_SUCCESS;
+}
+
+extern "C" int dm_split_twiddles(SplitPlan plan) {
+  if (plan == nullptr || plan->ext == nullptr) return -1;
+  return plan->ext->ctw ? DM_TWIDDLES_COMPENSATED : DM_TWIDDLES_PLAIN;
 }
 
 extern "C" int dm_split_phase_order(SplitPlan plan) {

@@ -115,6 +115,17 @@ SplitPlan makeParallelPlanf(Decomp start, int *initial_factor, fftwf_complex *da
 int dm_split_set_phase_order(SplitPlan plan, int on);
 /** 1 when the plan runs in phase order. */
 int dm_split_phase_order(SplitPlan plan);
+/** No counterpart in the reference (its transforms are complex double).
+ *  Twiddle products of a complex-float plan: DM_TWIDDLES_COMPENSATED (the
+ *  default) multiplies by each root as a float part plus a bfloat16
+ *  correction, formed from the FP64 root and added inside the fused
+ *  multiply-adds of the product, so the root is held to about 2^-33;
+ *  DM_TWIDDLES_PLAIN uses the float part alone (about 2^-25).  No effect on
+ *  complex-double plans.  Returns 0 or an error code. */
+enum { DM_TWIDDLES_COMPENSATED = 0, DM_TWIDDLES_PLAIN = 1 };
+int dm_split_set_twiddles(SplitPlan plan, int mode);
+/** The plan's twiddle mode, or -1 for a NULL plan. */
+int dm_split_twiddles(SplitPlan plan);
 /** No counterpart in the reference.  The forward result goes to
  *  work[0:extent) in the caller's decomposition `out` (compatible with the
  *  start, every coordinate owned, each rank's block within extent elements)

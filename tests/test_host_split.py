@@ -252,6 +252,34 @@ def test_phase_order_flag(C):
     C.dm_world_free(comm)
 
 
+def test_twiddle_mode_setting(C):
+    """dm_split_set_twiddles: compensated products by default, plain on
+    request, unknown modes and NULL plans refused."""
+    comm = C.dm_world_create(4)
+    start = C.makePackedDecomp(comm, 3, dm.ptrs([prod_dim(C, ("block", 16, 1)), prod_dim(C, ("block", 16, 2)),
+                                                 prod_dim(C, ("block", 16, 2))]))
+    per = C.extent(start)
+    dat, work = HostBuffers(C, comm, 4, per), HostBuffers(C, comm, 4, 4 * per + 64)
+    for make in (C.makeSplitPlanf, C.makeSplitPlan):
+        p = make(0, start, dat.ptr, 4 * per + 64, work.ptr)
+        assert p and C.dm_split_twiddles(p) == dm.DM_TWIDDLES_COMPENSATED
+        assert C.dm_split_set_twiddles(p, dm.DM_TWIDDLES_PLAIN) == 0
+        assert C.dm_split_twiddles(p) == dm.DM_TWIDDLES_PLAIN
+        C.dm_set_error_mode(1)
+        assert C.dm_split_set_twiddles(p, 7) != 0
+        C.dm_clear_error()
+        C.dm_set_error_mode(0)
+        assert C.dm_split_twiddles(p) == dm.DM_TWIDDLES_PLAIN
+        assert C.dm_split_set_twiddles(p, dm.DM_TWIDDLES_COMPENSATED) == 0
+        assert C.dm_split_twiddles(p) == dm.DM_TWIDDLES_COMPENSATED
+        C.freeSplitPlan(p)
+    assert C.dm_split_set_twiddles(None, 0) != 0 and C.dm_split_twiddles(None) == -1
+    dat.free()
+    work.free()
+    C.freeDecomp(start)
+    C.dm_world_free(comm)
+
+
 IO_CASES = [
     (6, [("block", 10, 2), ("rotate", 3, ("block", 7, 3)), ("local", 5)], None, None),
     (4, [("interleave", ("block", 4, 2), 3), ("block", 9, 2)], None, None),
