@@ -50,6 +50,7 @@ struct dm_batch {
   void *d_tw = nullptr;
   size_t lds = 0;
   int primes = 0;  // some schedule has a radix 5, 7, 11 or 13
+  int long_lines = 0;  // some schedule's twiddles stay in device memory
   std::vector<PassTables> passes;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -172,6 +173,7 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
     S.tw_off = (int)(h.tw.size() / 2);
     for (int s = 0; s < S.nstage; s++)
       if (S.radix[s] == 5 || S.radix[s] == 7 || S.radix[s] == 11 || S.radix[s] == 13) h.primes = 1;
+    if (!batch_tw_in_lds(N)) h.long_lines = 1;
     h.tw.insert(h.tw.end(), block.begin(), block.end());
     int id = (int)h.sched.size();
     h.sched.push_back(S);
@@ -234,6 +236,7 @@ static bool bind_tables(DmBatch b) {
   }
   b->lds = h.lds;
   b->primes = h.primes;
+  b->long_lines = h.long_lines;
   b->passes.assign(b->ndim, PassTables());
   for (int d = 0; d < b->ndim; d++) b->passes[d].tiles = std::move(h.tiles[d]);
   std::vector<dm::BatchSched> &sched = h.sched;
@@ -326,6 +329,7 @@ extern "C" int dm_batch_run(DmBatch b, void *stream) {
     L.dbl = b->dbl;
     L.inverse = b->inverse;
     L.primes = b->primes;
+    L.long_lines = b->long_lines;
     L.grid = p.grid;
     L.lds = b->lds;
     if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run: batched pass launch")) return 1;

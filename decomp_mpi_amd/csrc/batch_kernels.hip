@@ -23,6 +23,15 @@ using gptr = __attribute__((address_space(1))) cx<R> *;  // global memory
 constexpr int BCAP_D = 4096;
 constexpr int BCAP_F = 4096;  // 8 points per thread in both precisions
 constexpr int BNTH = 512;
+// Twiddles (16 bytes per root) go to LDS after the tile when every schedule
+// of the launch has lines of at most BTW_LDS_N points; otherwise the launch
+// reads them from device memory.
+constexpr int BTW_LDS_N = 256;
+__host__ __device__ __forceinline__ size_t batch_tw_at(int T, int N, int es) {
+  return ((size_t)T * (size_t)lds_line(N) * es + 15) & ~(size_t)15;
+}
+template <typename R>
+using lds_twg = __attribute__((address_space(3))) typename tw_global<R>::type;
 
 
 // Element access (the non-temporal hint measured slower here: a pass reads
@@ -94,10 +103,9 @@ __device__ __forceinline__ void load_first(cx<R> *buf, const BatchSched &S, cons
 
 // Last stage straight to memory: butterfly inputs from LDS, twiddles
 // (stage-ordered: Ns = nr in the last stage), transform, store.
-template <typename R, bool INV, int RADIX, int CAP, int NTH>
+template <typename R, bool INV, int RADIX, int CAP, int NTH, typename TW>
 __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S, const BatchTile &d,
-                                           gptr<R> data, long long lstride, int ls,
-                                           const typename tw_global<R>::type *tw) {
+                                           gptr<R> data, long long lstride, int ls, TW tw) {
   constexpr int K = (CAP / NTH) / RADIX;
   const StageInfo &st = S.st[S.nstage - 1];
   const int nr = st.nr, nb = S.T * nr;
@@ -127,11 +135,14 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 // SIMD: complex float fits 64 VGPRs without spills, so 8 waves and four
 // 512-thread workgroups per CU keep more loads in flight (81 VGPRs allowed
 // only two); complex double spills below 104, so it stays at 4.
-template <typename R, bool INV, int CAP, int NTH, bool PRIMES>
+// TWL: the schedules' twiddles are copied to LDS per tile (every line of
+// the launch at most BTW_LDS_N points).
+template <typename R, bool INV, int CAP, int NTH, bool PRIMES, bool TWL>
 __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel(BatchLaunch p) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   constexpr int V = CAP / NTH;
+  constexpr bool CT = sizeof(R) == 4;  // complex float: compensated twiddle products
   for (long long t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
     // the tile, its schedule, roots and transform: from the launch's tables,
     // or from the segment (one transform's own tables) that holds tile t
@@ -179,6 +190,9 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     // stated: through a generic pointer every access would be a flat one
     gptr<R> data = (gptr<R>)(xdata) + d.base;
     const TwE *tw = twall + S.tw_off;
+    TwE *twl = reinterpret_cast<TwE *>(smem_raw + batch_tw_at(T, N, sizeof(cx<R>)));
+    if constexpr (TWL)
+      for (int i = threadIdx.x; i < N; i += NTH) twl[i] = tw[i];
     // Fused first / last stages: a power-of-two radix dividing the points per
     // thread (whole butterflies per thread); the last one needs two stages.
     const int r0 = S.radix[0], rl = S.radix[S.nstage - 1];
@@ -222,25 +236,30 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     }
     __syncthreads();
     const int s0 = fuse_in ? 1 : 0, s1 = fuse_out ? S.nstage - 1 : S.nstage;
-    for (int s = s0; s < s1; s++) {
-      switch (S.radix[s]) {
-        case 2: stage_lds<R, INV, 2, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 3: stage_lds<R, INV, 3, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 4: stage_lds<R, INV, 4, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 5: if constexpr (PRIMES) stage_lds<R, INV, 5, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 7: if constexpr (PRIMES) stage_lds<R, INV, 7, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 8: stage_lds<R, INV, 8, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        case 16: stage_lds<R, INV, 16, CAP, NTH, (sizeof(R) == 4)>(buf, S.st[s], T, ls, tw); break;
-        default: break;
+    auto run_stages = [&](auto tw) {
+      for (int s = s0; s < s1; s++) {
+        switch (S.radix[s]) {
+          case 2: stage_lds<R, INV, 2, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 3: stage_lds<R, INV, 3, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 4: stage_lds<R, INV, 4, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 5: if constexpr (PRIMES) stage_lds<R, INV, 5, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 7: if constexpr (PRIMES) stage_lds<R, INV, 7, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 8: stage_lds<R, INV, 8, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          case 16: stage_lds<R, INV, 16, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
+          default: break;
+        }
       }
-    }
-    if (fuse_out) {
-      if (rl == 8) last_store<R, INV, 8, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
-      else if (rl == 4) last_store<R, INV, 4, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
-      else last_store<R, INV, 2, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
-    } else {
+      if (fuse_out) {
+        if (rl == 8) last_store<R, INV, 8, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
+        else if (rl == 4) last_store<R, INV, 4, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
+        else last_store<R, INV, 2, CAP, NTH>(buf, S, d, data, lstride, ls, tw);
+      }
+    };
+    if constexpr (TWL) run_stages((const lds_twg<R> *)twl);
+    else run_stages(tw);
+    if (!fuse_out) {
       // store: same element walk as the load, straight from LDS
 #pragma unroll
       for (int k = 0; k < V; k++) {
@@ -265,10 +284,10 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
   }
 }
 
-template <typename R, bool INV, int CAP, bool PRIMES>
+template <typename R, bool INV, int CAP, bool PRIMES, bool TWL>
 hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
   constexpr int NTH = BNTH;
-  auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES>;
+  auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL>;
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -283,18 +302,20 @@ hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
 
 int batch_tile_capacity(int dbl) { return dbl ? BCAP_D : BCAP_F; }
 
+bool batch_tw_in_lds(int N) { return N <= BTW_LDS_N; }
+
 size_t batch_lds_bytes(int dbl, int T, int N) {
-  return (size_t)T * (size_t)lds_line(N) * (dbl ? 16 : 8);
+  return batch_tw_at(T, N, dbl ? 16 : 8) + (batch_tw_in_lds(N) ? (size_t)N * 16 : 0);
 }
 
 // Workgroups resident at once: by LDS (160 KiB per CU) and by registers
 // (512 VGPRs per SIMD lane, 4 SIMDs per CU, at most 8 waves per SIMD).
 long long batch_resident_groups(int dbl, int primes, size_t lds) {
   const void *k;
-  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true>
-                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false>;
-  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true>
-                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false>;
+  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true, true>
+                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false, true>;
+  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true, true>
+                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false, true>;
   hipFuncAttributes fa;
   int vgprs = (hipFuncGetAttributes(&fa, k) == hipSuccess && fa.numRegs > 0) ? fa.numRegs : 128;
   int dev = 0, cus = 0;
@@ -307,17 +328,18 @@ long long batch_resident_groups(int dbl, int primes, size_t lds) {
   return (long long)cus * std::max(1, std::min(by_regs, by_lds));
 }
 
+template <typename R, int CAP, bool TWL>
+hipError_t launch_forms(const BatchLaunch &p, hipStream_t s) {
+  if (p.primes) return p.inverse ? launch_typed<R, true, CAP, true, TWL>(p, s) : launch_typed<R, false, CAP, true, TWL>(p, s);
+  return p.inverse ? launch_typed<R, true, CAP, false, TWL>(p, s) : launch_typed<R, false, CAP, false, TWL>(p, s);
+}
+
 int launch_batch_pass(const BatchLaunch &p, void *stream) {
   if (p.ntiles <= 0 || p.grid <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
-  if (p.dbl) {
-    if (p.primes) e = p.inverse ? launch_typed<double, true, BCAP_D, true>(p, s) : launch_typed<double, false, BCAP_D, true>(p, s);
-    else e = p.inverse ? launch_typed<double, true, BCAP_D, false>(p, s) : launch_typed<double, false, BCAP_D, false>(p, s);
-  } else {
-    if (p.primes) e = p.inverse ? launch_typed<float, true, BCAP_F, true>(p, s) : launch_typed<float, false, BCAP_F, true>(p, s);
-    else e = p.inverse ? launch_typed<float, true, BCAP_F, false>(p, s) : launch_typed<float, false, BCAP_F, false>(p, s);
-  }
+  if (p.dbl) e = p.long_lines ? launch_forms<double, BCAP_D, false>(p, s) : launch_forms<double, BCAP_D, true>(p, s);
+  else e = p.long_lines ? launch_forms<float, BCAP_F, false>(p, s) : launch_forms<float, BCAP_F, true>(p, s);
   return (int)e;
 }
 

@@ -144,6 +144,7 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
     L.dbl = dbl;
     L.inverse = direction == 1 ? 1 : 0;
     L.primes = h.primes;
+    L.long_lines = h.long_lines;
     L.grid = L.ntiles;  // one workgroup per tile, as the bound form's default
     L.lds = h.lds;
     if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run_noalloc: batched pass launch"))

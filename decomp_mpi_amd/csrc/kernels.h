@@ -163,6 +163,7 @@ struct BatchLaunch {
   const void *twiddle;         // all schedules' roots in the pass precision
   int dbl, inverse;
   int primes;                  // some schedule has a radix 5, 7, 11 or 13
+  int long_lines;              // some schedule's lines are too long for twiddles in LDS
   long long grid;              // persistent workgroups (tile loop inside)
   size_t lds;
   int nseg;                    // > 0: tiles come from seg[] (tiles, sched, data, twiddle unused)
@@ -178,6 +179,7 @@ struct BatchHost {
   std::vector<double> tw;
   size_t lds = 0;
   int primes = 0;
+  int long_lines = 0;
 };
 bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_choice, int radix_order,
                       BatchHost *out);
@@ -186,7 +188,11 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
 void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vector<double> *tw);
 int batch_tile_capacity(int dbl);
 int batch_tile_height(int N, long long inner, int dbl, int tile_choice);
+// LDS of a tile of T lines of N points, with the schedule's twiddles when
+// batch_tw_in_lds(N) (a launch copies them only if that holds for all its
+// schedules: BatchLaunch::long_lines 0).
 size_t batch_lds_bytes(int dbl, int T, int N);
+bool batch_tw_in_lds(int N);
 long long batch_resident_groups(int dbl, int primes, size_t lds);
 
 }  // namespace dm

@@ -56,6 +56,7 @@ struct Sched {
   const void *tw = nullptr;
   size_t lds = 0;
   int primes = 0;
+  int long_lines = 0;       // lines too long for twiddles in LDS
   int refs = 0;             // requests admitted with this schedule and not yet finished
   long long last_use = 0;
 };
@@ -69,6 +70,7 @@ struct ReqPlan {
   long long ntiles[3] = {0, 0, 0};
   size_t lds = 0;
   int primes = 0;
+  int long_lines = 0;
 };
 
 struct Req {
@@ -213,6 +215,7 @@ static Sched *sched_for(DmServer s, int N, int T, int dbl) {
   p->sched = (const dm::BatchSched *)p->dev;
   p->tw = (char *)p->dev + o_tw;
   p->lds = dm::batch_lds_bytes(dbl, T, N);
+  p->long_lines = dm::batch_tw_in_lds(N) ? 0 : 1;
   for (int k = 0; k < one.nstage; k++)
     if (one.radix[k] == 5 || one.radix[k] == 7 || one.radix[k] == 11 || one.radix[k] == 13) p->primes = 1;
   p->last_use = ++s->sched_clock;
@@ -270,6 +273,7 @@ static bool plan_request(DmServer s, Req &r) {
     }
     pl.lds = std::max(pl.lds, sc->lds);
     pl.primes = pl.primes || sc->primes;
+    pl.long_lines = pl.long_lines || sc->long_lines;
   }
   r.planned = true;
   return true;
@@ -407,16 +411,18 @@ static bool launch(DmServer s, Slot &sl) {
   }
   ok = ok && hipEventRecord(sl.ev[1], st) == hipSuccess;
   size_t lds = 0;
-  int primes = 0;
+  int primes = 0, long_lines = 0;
   for (long long id : ids) {
     lds = std::max(lds, s->reqs[id].plan.lds);
     primes = primes || s->reqs[id].plan.primes;
+    long_lines = long_lines || s->reqs[id].plan.long_lines;
   }
   for (int d = 0; d < nd && ok; d++) {
     dm::BatchLaunch L{};
     L.dbl = dbl;
     L.inverse = inv;
     L.primes = primes;
+    L.long_lines = long_lines;
     L.lds = lds;
     L.nseg = (int)ids.size();
     long long end = 0;
