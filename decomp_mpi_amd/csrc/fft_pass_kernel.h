@@ -72,6 +72,7 @@ struct LineAddr<true> {
   }
   __device__ __forceinline__ i64 at(const FftLaunch &, const FftSide &s, const i64 *absT, int l, int x) const {
     if (s.aff) return base[l] + (i64)x * s.aff;  // affine along t: no table load
+    if (s.jump) return base[l] + (i64)(x >> s.rsh) * s.jump + (i64)(x & s.rmask) * s.rstride;
     return base[l] + absT[x];
   }
   // the same with the side's form fixed at compile time (no branch)

@@ -221,14 +221,37 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     fs.uniform = s.h.uniform ? 1 : 0;
     fs.proc_used = s.h.proc_used;
     fs.slot = s.slot;
-    // offsets along t affine (x * m_t: axis t local to every rank, in order)
+    // offsets along t affine (x * m_t: axis t local to every rank, in
+    // order), or in runs of R = 2^k points at a jump J per run (axis t in
+    // equal blocks over the ranks: (x >> k) * J + (x & (R - 1)) * m_t)
     fs.aff = 0;
+    fs.jump = 0;
     if (s.h.uniform && s.h.m[t] > 0) {
       bool affine = true;
       for (int x = 0; x < s.h.L[t] && affine; x++) affine = s.h.rank_part[t][x] == 0 && s.h.off[t][x] == x;
       static int noaff = -1;
       if (noaff < 0) noaff = getenv("DM_NO_AFFINE") != nullptr;
       if (affine && !noaff) fs.aff = s.h.m[t];
+      const std::vector<i64> &a = s.abs[t];
+      const i64 m = s.h.m[t];
+      const int n = s.h.L[t];
+      if (!affine && !noaff && a[0] == 0) {
+        int R = 1;
+        while (R < n && a[R] == a[R - 1] + m) R++;
+        if (R < n && (R & (R - 1)) == 0) {
+          int k = 0;
+          while ((1 << k) < R) k++;
+          const i64 J = a[R];
+          bool ok = J != 0;
+          for (int x = 0; x < n && ok; x++) ok = a[x] == (i64)(x >> k) * J + (i64)(x & (R - 1)) * m;
+          if (ok) {
+            fs.jump = J;
+            fs.rstride = m;
+            fs.rsh = k;
+            fs.rmask = R - 1;
+          }
+        }
+      }
     }
     if (s.h.uniform) return;
     for (int d = 0; d < nd; d++) {
@@ -312,7 +335,11 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   pp->tw = (long)tb.put(img.data(), img.size());
   L.in = in;
   L.out = out;
-  auto form = [](const FftSide &f) { return std::string(f.aff ? " affine" : ""); };
+  auto form = [](const FftSide &f) {
+    if (f.aff) return std::string(" affine");
+    if (f.jump) return " affine in runs of " + std::to_string(f.rmask + 1);
+    return std::string("");
+  };
   logger(comm, "FFT pass: axis %d (N %d, %d stages) lines along %d, T %d, %lld workgroups, in %s stride %lld%s, out %s stride %lld%s%s%s\n",
          t, L.N, L.nstage, u, L.T, (long long)L.tiles_u * batch, in_row ? "row" : "column", (long long)I.h.m[t],
          form(L.sin).c_str(), out_row ? "row" : "column", (long long)U.h.m[t], form(L.sout).c_str(),

@@ -52,7 +52,9 @@ struct FftSide {
   int uniform;
   int proc_used;
   i64 slot;                    // elements per rank slot
-  i64 aff;                     // uniform side whose offset along t is x * aff (0: use the table)
+  i64 aff;                     // uniform side whose offset along t is x * aff (0: see jump)
+  i64 jump, rstride;           // aff 0, jump != 0: runs of rmask + 1 points, offset
+  int rsh, rmask;              //   (x >> rsh) * jump + (x & rmask) * rstride; else the table
   const int *rank[DM_MAX_DIMS];
   const int *off[DM_MAX_DIMS];
   const i64 *strides;          // [proc_used * ndim]
