@@ -18,7 +18,7 @@ namespace {
 constexpr int TILE = 32;      // tile edge (elements)
 constexpr int ROWS = 8;       // thread rows per tile pass
 constexpr int NTHREADS = TILE * ROWS;
-constexpr int TILE_SLICES = 4;   // batch slices per 2D-tile workgroup
+constexpr int TILE_SLICES = 2;   // batch slices per 2D-tile workgroup (4 measured 5% slower, 1 20%)
 
 template <int BYTES>
 struct Elem;
@@ -302,17 +302,22 @@ int launch_reshape(const ReshapeLaunch &p0, void *stream) {
 // ---- plain copy, the calibration reference for bandwidth and counters ------
 namespace dm {
 namespace {
+// One 16-byte element per thread, 4 KiB per workgroup, non-temporal: the
+// fastest plain copy measured here (6.4 TB/s for 2 GiB; 16 KiB per
+// workgroup without the hint: 5.6-5.8 TB/s).
 __global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                      long long n) {
-  long long i = (long long)blockIdx.x * 256 * 4 + threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < 4; k++, i += 256)
-    if (i < n) dst[i] = src[i];
+  typedef unsigned int v4 __attribute__((ext_vector_type(4)));
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(src) + i);
+    __builtin_nontemporal_store(v, reinterpret_cast<v4 *>(dst) + i);
+  }
 }
 }  // namespace
 
 int launch_copy16(const void *src, void *dst, long long n16, void *stream) {
-  long long blocks = (n16 + 1023) / 1024;
+  long long blocks = (n16 + 255) / 256;
   if (blocks <= 0) return 0;
   hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      (const uint4 *)src, (uint4 *)dst, n16);
