@@ -16,30 +16,33 @@ namespace dm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPerThread = 4;
+constexpr int kPerThread = 2;
 
-template <typename R, bool CONJ>
+// IDX: the integer type of a rank's element index (32-bit unless a rank
+// holds 2^32 elements or more: a 64-bit division costs several times more).
+template <typename R, bool CONJ, typename IDX>
 __global__ __launch_bounds__(kThreads) void scale_kernel(ScaleLaunch p) {
   const int r = blockIdx.y;
   const int nd = p.ndim;
   const int *rt = p.rank_tab + (size_t)r * nd * 2;
   const i64 *st = p.strides + (size_t)r * nd;
-  long long total = 1;
-  for (int d = 0; d < nd; d++) total *= rt[2 * d + 1];
+  IDX total = 1;
+  for (int d = 0; d < nd; d++) total *= (IDX)rt[2 * d + 1];
   typedef R v2 __attribute__((ext_vector_type(2)));
   v2 *data = reinterpret_cast<v2 *>(p.data) + (i64)r * p.slot;
-  const long long base = (long long)blockIdx.x * kThreads * kPerThread + threadIdx.x;
+  const IDX base = (IDX)blockIdx.x * kThreads * kPerThread + threadIdx.x;
 #pragma unroll
   for (int k = 0; k < kPerThread; k++) {
-    long long e = base + (long long)k * kThreads;
+    IDX e = base + (IDX)k * kThreads;
     if (e >= total) break;
     // local coordinates, last axis fastest
     i64 off = 0;
     int loc[DM_MAX_DIMS];
     for (int d = nd - 1; d >= 0; d--) {
-      const int c = rt[2 * d + 1];
-      loc[d] = (int)(e % c);
-      e /= c;
+      const IDX c = (IDX)rt[2 * d + 1];
+      const IDX q = e / c;
+      loc[d] = (int)(e - q * c);
+      e = q;
       off += (i64)loc[d] * st[d];
     }
     double fr = 1.0, fi = 0.0;
@@ -52,9 +55,10 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(ScaleLaunch p) {
     }
     if (fr == 0.0 && fi == 0.0) continue;
     if (CONJ) fi = -fi;
-    const v2 x = data[off];
+    // each element is read once and written once: non-temporal accesses
+    const v2 x = __builtin_nontemporal_load(data + off);
     const double xr = (double)x.x, xi = (double)x.y;
-    data[off] = v2{(R)(fr * xr - fi * xi), (R)(fr * xi + fi * xr)};
+    __builtin_nontemporal_store(v2{(R)(fr * xr - fi * xi), (R)(fr * xi + fi * xr)}, data + off);
   }
 }
 
@@ -65,12 +69,20 @@ int launch_scale(const ScaleLaunch &p, void *stream) {
   const long long per_block = (long long)kThreads * kPerThread;
   dim3 grid((unsigned)((p.max_elems + per_block - 1) / per_block), (unsigned)p.nranks);
   hipStream_t s = (hipStream_t)stream;
-  if (p.dbl) {
-    if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true>), grid, dim3(kThreads), 0, s, p);
-    else hipLaunchKernelGGL((scale_kernel<double, false>), grid, dim3(kThreads), 0, s, p);
+  if (p.max_elems + per_block >= (1LL << 32)) {
+    if (p.dbl) {
+      if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true, unsigned long long>), grid, dim3(kThreads), 0, s, p);
+      else hipLaunchKernelGGL((scale_kernel<double, false, unsigned long long>), grid, dim3(kThreads), 0, s, p);
+    } else {
+      if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true, unsigned long long>), grid, dim3(kThreads), 0, s, p);
+      else hipLaunchKernelGGL((scale_kernel<float, false, unsigned long long>), grid, dim3(kThreads), 0, s, p);
+    }
+  } else if (p.dbl) {
+    if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true, unsigned>), grid, dim3(kThreads), 0, s, p);
+    else hipLaunchKernelGGL((scale_kernel<double, false, unsigned>), grid, dim3(kThreads), 0, s, p);
   } else {
-    if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true>), grid, dim3(kThreads), 0, s, p);
-    else hipLaunchKernelGGL((scale_kernel<float, false>), grid, dim3(kThreads), 0, s, p);
+    if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true, unsigned>), grid, dim3(kThreads), 0, s, p);
+    else hipLaunchKernelGGL((scale_kernel<float, false, unsigned>), grid, dim3(kThreads), 0, s, p);
   }
   return (int)hipGetLastError();
 }
