@@ -493,6 +493,9 @@ def main():
         else:
             comm = C.dm_world_create(P)
             res = reshape_chain(C, dm, comm)
+            cb = dm.DistBuffer(comm, (2 << 30) // P)
+            res["copy_1GiB_ms"] = round(C.dm_copy_bench(comm, cb.ptr + (1 << 30), cb.ptr, 1 << 30, 3), 4)
+            cb.free()
         print(json.dumps({"section": args.only, **res}), flush=True)
         return
     print(f"bench: generating {L}^3 input", file=sys.stderr, flush=True)
