@@ -92,6 +92,30 @@ def test_server_results_match_numpy(C):
     release(reqs)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_server_lines_longer_than_lds_twiddles(C, prec):
+    """Requests with lines of more than 256 points (their batches read the
+    twiddles from device memory) mixed with short-line ones (twiddles in
+    LDS when batched alone), both directions, against numpy."""
+    rng = np.random.default_rng(33)
+    s = C.dm_server_create(8 << 30, 2, 8)
+    reqs = []
+    for shape, direction in (((512, 16, 16), -1), ((32, 48, 64), -1), ((16, 1024, 8), 1), ((64, 32, 32), 1)):
+        x = workloads.uniform_complex(rng, shape, prec)
+        reqs.append((shape, prec, direction, x, Pinned(C, x), Pinned(C, np.zeros_like(x))))
+    print(f"serve long lines: precision {prec}", flush=True)
+    for r in reqs:
+        submit(C, s, r)
+    assert C.dm_server_drain(s) == 0
+    for r in reqs:
+        got = r[5].array().astype(np.complex128)
+        w = want(r)
+        err = np.linalg.norm(got - w) / np.linalg.norm(w)
+        assert err < TOL[r[1]], (r[0], r[2], err)
+    C.dm_server_destroy(s)
+    release(reqs)
+
+
 def test_server_alone_equals_mix(C):
     probe = make_requests(C, 5, seed=31)
     alone = []
