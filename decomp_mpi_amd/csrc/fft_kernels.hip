@@ -92,14 +92,13 @@ int launch_fft_pass(const FftLaunch &p, void *stream) {
     P.st[s].m_ns = magic_for(ns);
     ns *= r;
   }
-  long long batch = 1;
-  for (int k = 0; k < p.nbatch; k++) batch *= p.L[p.batch_axis[k]];
+  const long long all = fft_pass_tiles(p);
   if (p.nblocks <= 0) {  // the whole pass
-    P.p.nblocks = (long long)p.tiles_u * batch;
+    P.p.nblocks = all;
     P.p.tile_base = 0;
   }
   if (P.p.nblocks <= 0) return 0;
-  if (P.p.tile_base < 0 || P.p.tile_base + P.p.nblocks > (long long)p.tiles_u * batch) return (int)hipErrorInvalidValue;
+  if (P.p.tile_base < 0 || P.p.tile_base + P.p.nblocks > all) return (int)hipErrorInvalidValue;
   size_t lds = fft_lds_bytes(p);
   hipStream_t s = (hipStream_t)stream;
   bool pow2 = true;

@@ -120,7 +120,8 @@ __device__ __forceinline__ int setup_lines(const FftLaunch &p, long long tile, L
                                            LineAddr<UOUT> &lout, int T, int NTHR) {
   int tu = (int)(tile % p.tiles_u);
   const int u0 = tu * T;
-  const int nlines = p.u >= 0 ? min(T, p.L[p.u] - u0) : 1;
+  const int mk = p.merge_k;
+  const int nlines = mk > 1 ? T : (p.u >= 0 ? min(T, p.L[p.u] - u0) : 1);
   if constexpr (UIN && UOUT) {
     if (p.line_in_tab != nullptr) {  // precomputed line bases: one load each
       for (int l = threadIdx.x; l < T; l += NTHR) {
@@ -135,14 +136,25 @@ __device__ __forceinline__ int setup_lines(const FftLaunch &p, long long tile, L
   for (int d = 0; d < DM_MAX_DIMS; d++) gb[d] = 0;
   for (int k = p.nbatch - 1; k >= 0; k--) {
     int d = p.batch_axis[k];
-    gb[d] = (int)(bid % p.L[d]);
-    bid /= p.L[d];
+    const bool merged = mk > 1 && d == p.merge_v;
+    const int len = merged ? p.L[d] / mk : p.L[d];
+    const int c = (int)(bid % len);
+    gb[d] = merged ? c * mk : c;
+    bid /= len;
   }
   for (int l = threadIdx.x; l < T; l += NTHR) {
     int g[DM_MAX_DIMS];
     for (int d = 0; d < p.ndim; d++) g[d] = gb[d];
     g[p.t] = 0;
-    if (p.u >= 0) g[p.u] = (l < nlines) ? (p.ord_u ? p.ord_u[u0 + l] : u0 + l) : 0;
+    if (mk > 1) {
+      // all L[u] lines for each of the tile's merge_k values of axis merge_v
+      const int lu = p.L[p.u];
+      const int cu = l % lu;
+      g[p.u] = p.ord_u ? p.ord_u[cu] : cu;
+      g[p.merge_v] += l / lu;
+    } else if (p.u >= 0) {
+      g[p.u] = (l < nlines) ? (p.ord_u ? p.ord_u[u0 + l] : u0 + l) : 0;
+    }
     if constexpr (UIN) lin.set_line(p, p.off_in, g, l);
     else lin.set_line(p, p.sin, g, l);
     if constexpr (UOUT) lout.set_line(p, p.off_out, g, l);

@@ -203,10 +203,33 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   L.nbatch = 0;
   for (int d = 0; d < nd; d++)
     if (d != t && d != u) L.batch_axis[L.nbatch++] = d;
+  // A row walk on both sides whose u axis holds less than half a tile of
+  // lines: each tile takes all L[u] lines for k consecutive values of the
+  // fastest batch axis (k dividing its length), so tiles stay full (the
+  // sub-axis passes of the phase order: 16- or 32-point lines, 16 or 32 of
+  // them along u).
+  L.merge_v = -1;
+  L.merge_k = 1;
+  if (u >= 0 && L.nbatch > 0 && in_row && out_row) {
+    const int per = std::min(fft_tile_capacity(dbl, 0) / L.N, 512);
+    const int lu = I.h.L[u];
+    const int v = L.batch_axis[L.nbatch - 1];
+    if (2 * lu <= per) {
+      for (int k = per / lu; k >= 2; k--)
+        if (I.h.L[v] % k == 0) {
+          L.merge_v = v;
+          L.merge_k = k;
+          L.T = lu * k;
+          L.tiles_u = 1;
+          break;
+        }
+    }
+  }
+  const long long ntiles = fft_pass_tiles(L);
   long long batch = 1;
   for (int k = 0; k < L.nbatch; k++) batch *= I.h.L[L.batch_axis[k]];
-  if ((long long)L.tiles_u * batch >= (1LL << 31)) {
-    raise_error(comm, "FFT: %lld workgroups exceed the launch limit", (long long)L.tiles_u * batch);
+  if (ntiles >= (1LL << 31)) {
+    raise_error(comm, "FFT: %lld workgroups exceed the launch limit", ntiles);
     return false;
   }
   L.inverse = inverse ? 1 : 0;
@@ -273,7 +296,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   // the kernel instead of a chain of lookups (bounded to 64 MiB of tables).
   {
     long long nlines = (long long)L.tiles_u * batch * L.T;
-    if (I.h.uniform && U.h.uniform && nlines * 16 <= (64LL << 20)) {
+    if (I.h.uniform && U.h.uniform && L.merge_k == 1 && nlines * 16 <= (64LL << 20)) {
       std::vector<i64> bi(nlines), bo(nlines);
       const std::vector<int> *ou = nullptr;
       if (pp->ord_u >= 0) ou = &((out_row ? I : U).order[u]);
@@ -341,7 +364,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     return std::string("");
   };
   logger(comm, "FFT pass: axis %d (N %d, %d stages) lines along %d, T %d, %lld workgroups, in %s stride %lld%s, out %s stride %lld%s%s%s\n",
-         t, L.N, L.nstage, u, L.T, (long long)L.tiles_u * batch, in_row ? "row" : "column", (long long)I.h.m[t],
+         t, L.N, L.nstage, u, L.T, ntiles, in_row ? "row" : "column", (long long)I.h.m[t],
          form(L.sin).c_str(), out_row ? "row" : "column", (long long)U.h.m[t], form(L.sout).c_str(),
          L.stream_in ? ", non-temporal loads" : "", L.stream_out ? ", non-temporal stores" : "");
   return true;

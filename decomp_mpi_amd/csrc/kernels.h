@@ -73,6 +73,8 @@ struct FftLaunch {
   int nbatch;
   int batch_axis[DM_MAX_DIMS];
   int tiles_u;
+  int merge_v, merge_k;        // merge_k > 1: a tile holds all L[u] lines for merge_k consecutive
+                               //   values of batch axis merge_v (tiles_u 1; merge_k divides L[v])
   long long nblocks;
   const i64 *off_in[DM_MAX_DIMS];
   const i64 *off_out[DM_MAX_DIMS];
@@ -93,6 +95,16 @@ struct FftLaunch {
   int stream_in, stream_out;  // 1: non-temporal hint on the data loads / stores
   int ctw;                    // complex float: 1 compensated twiddle products, 0 the float part only
 };
+// Tiles of a whole pass: tiles along u times the batch, the merged axis
+// counting in blocks of merge_k.
+inline long long fft_pass_tiles(const FftLaunch &p) {
+  long long n = p.tiles_u;
+  for (int k = 0; k < p.nbatch; k++) {
+    const int d = p.batch_axis[k];
+    n *= (p.merge_k > 1 && d == p.merge_v) ? p.L[d] / p.merge_k : p.L[d];
+  }
+  return n;
+}
 // nblocks == 0: the launch covers every tile of the pass.
 int launch_fft_pass(const FftLaunch &p, void *stream);
 // Largest points per tile the FFT pass kernel holds for an element size:
