@@ -169,6 +169,88 @@ static void extend_by_words(dm::HostLayout &h, int words) {
   h.ndim = nd + 1;
 }
 
+// Axis d (n = Q * P points) as two axes: z = g / P stays at d (Q points) and
+// y = g % P is appended (P points), when both the owner contribution and the
+// local offset of g split as a part of z plus a part of y that is zero at
+// y = 0 -- for example a block axis (runs of P or a multiple of P) or an
+// interleave of period P.  Returns false (h untouched) otherwise.
+static bool separable(const dm::HostLayout &h, int d, int P) {
+  const int n = h.L[d];
+  const std::vector<int> &rp = h.rank_part[d], &of = h.off[d];
+  for (int g = 0; g < n; g++) {
+    const int z = g / P, y = g % P;
+    if (rp[g] >= (1 << 27)) return false;  // a coordinate without owner
+    if (rp[g] != rp[z * P] + rp[y] - rp[0] || of[g] != of[z * P] + of[y] - of[0]) return false;
+    if (rp[y] < rp[0] || of[y] < of[0]) return false;
+  }
+  return true;
+}
+
+static void split_axis(dm::HostLayout &h, int d, int P) {
+  const int n = h.L[d], Q = n / P, nd = h.ndim;
+  std::vector<int> rz(Q), oz(Q), ry(P), oy(P);
+  for (int z = 0; z < Q; z++) {
+    rz[z] = h.rank_part[d][z * P];
+    oz[z] = h.off[d][z * P];
+  }
+  for (int y = 0; y < P; y++) {
+    ry[y] = h.rank_part[d][y] - h.rank_part[d][0];
+    oy[y] = h.off[d][y] - h.off[d][0];
+  }
+  h.L[d] = Q;
+  h.rank_part[d] = rz;
+  h.off[d] = oz;
+  h.L.push_back(P);
+  h.rank_part.push_back(ry);
+  h.off.push_back(oy);
+  std::vector<i64> s;
+  for (int r = 0; r < h.proc_used; r++) {
+    for (int e = 0; e < nd; e++) s.push_back(h.strides[(size_t)r * nd + e]);
+    s.push_back(h.strides[(size_t)r * nd + d]);
+  }
+  h.strides = s;
+  h.m.push_back(h.m[d]);
+  h.ndim = nd + 1;
+}
+
+// Whether consecutive coordinates of axis d step by one local offset within
+// one owner (the axis is read or written in whole runs).
+static bool unit_steps(const dm::HostLayout &h, int d, int from, int step, int count) {
+  for (int k = 1; k < count; k++) {
+    const int g = from + k * step, p = g - step;
+    if (h.rank_part[d][g] != h.rank_part[d][p] || h.off[d][g] != h.off[d][p] + 1) return false;
+  }
+  return true;
+}
+
+// Both sides share their fastest axis d but not its order along it (for
+// example a block axis against an interleaved one): the gather would walk
+// 1D runs with scattered stores.  Split d into (z, y) with the source
+// contiguous along y and the destination along z, so the reshape becomes a
+// 2D tile transpose over (z, y).  The period P is the largest divisor of
+// the axis, up to sqrt of its length, that separates both sides that way
+// (or the smallest above it).
+static bool split_shared_axis(dm_reshape_program *pg) {
+  dm::HostLayout &S = pg->src, &D = pg->dst;
+  const int d = S.fastest;
+  if (D.fastest != d || S.ndim + 2 > DM_MAX_DIMS) return false;
+  const int n = S.L[d];
+  if (n < 4 || (unit_steps(S, d, 0, 1, n) && unit_steps(D, d, 0, 1, n))) return false;
+  int best = 0;
+  for (int P = 2; P <= n / 2; P++) {
+    if (n % P != 0 || !separable(S, d, P) || !separable(D, d, P)) continue;
+    // source in runs along y, destination in runs along z
+    if (!unit_steps(S, d, 0, 1, P) || !unit_steps(D, d, 0, P, n / P)) continue;
+    if (best == 0 || std::min(P, n / P) > std::min(best, n / best)) best = P;
+  }
+  if (best == 0) return false;
+  split_axis(S, d, best);
+  split_axis(D, d, best);
+  S.fastest = S.ndim - 1;  // y
+  D.fastest = d;           // z
+  return true;
+}
+
 // Relabel the global coordinates of axis d on both sides so that they run in
 // `key`'s memory order (owner rank, then local offset).  A bijection applied
 // to both tables keeps the pairing by global coordinate, and a tile over the
@@ -260,15 +342,18 @@ extern "C" ReshapePlan makeReshapePlan(MPI_Comm comm, MPI_Datatype base, Decomp 
     freeReshapePlan(plan);
     return nullptr;
   }
+  const bool split = getenv("DM_NO_SPLIT") == nullptr && split_shared_axis(pg);
+  const int nd_el = pg->src.ndim;  // element axes (a split adds one)
   extend_by_words(pg->src, pg->words);
   extend_by_words(pg->dst, pg->words);
   pg->a = pg->dst.fastest;
   pg->b = pg->src.fastest;
-  if (pg->words > 1) {  // tile over the element axes, not the word axis
+  // (a split keeps its two halves as the tile axes, also over words)
+  if (!split && pg->words > 1) {  // tile over the element axes, not the word axis
     auto pick = [&](const dm::HostLayout &h) {
       int best = 0;
       i64 bs = -1;
-      for (int d = ndim - 1; d >= 0; d--)
+      for (int d = nd_el - 1; d >= 0; d--)
         if (h.L[d] > 1 && (bs < 0 || h.m[d] < bs)) bs = h.m[d], best = d;
       return best;
     };

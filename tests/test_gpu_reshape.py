@@ -141,6 +141,34 @@ def test_descriptor_zoo_bit_exact(C, seed):
     assert np.array_equal(got, want)
 
 
+# Both sides share their fastest axis but not its order along it: block
+# against interleave (the split FFT's start -> A and A -> B reshapes), with
+# rotation, uneven blocks and a factor-reversed order.
+SAME_FASTEST = [
+    (4, [("block", 8, 2), ("local", 6), ("block", 32, 2)],
+     [("block", 8, 2), ("local", 6), ("interleave", ("block", 4, 2), 8)]),
+    (8, [("local", 4), ("block", 12, 2), ("block", 48, 4)],
+     [("local", 4), ("block", 12, 2), ("interleave", ("block", 8, 4), 6)]),
+    (6, [("block", 6, 3), ("local", 5), ("rotate", 3, ("block", 20, 2))],
+     [("block", 6, 3), ("local", 5), ("interleave", ("block", 4, 2), 5)]),
+    (4, [("local", 3), ("interleave", ("block", 4, 2), 6), ("interleave", ("block", 8, 2), 4)],
+     [("local", 3), ("block", 24, 2), ("block", 32, 2)]),
+    (6, [("block", 5, 3), ("local", 7), ("frev", 4, 9, ("block", 36, 2))],
+     [("block", 5, 3), ("local", 7), ("block", 36, 2)]),
+    (4, [("local", 2), ("block", 10, 2), ("block", 50, 2)],
+     [("local", 2), ("block", 10, 2), ("interleave", ("block", 10, 2), 5)]),
+]
+
+
+@pytest.mark.parametrize("elem", [4, 8, 16])
+@pytest.mark.parametrize("case", range(len(SAME_FASTEST)))
+def test_same_fastest_axis_bit_exact(C, case, elem):
+    P, sd, dd = SAME_FASTEST[case]
+    print(f"same fastest axis case {case}, {elem}-byte elements", flush=True)
+    got, want = run_case(C, P, sd, dd, elem, seed=case)
+    assert np.array_equal(got, want)
+
+
 def pencil_descs(n, local_axis, grid=(2, 2)):
     it = iter(grid)
     return [("local", n) if d == local_axis else ("block", n, next(it)) for d in range(3)]
