@@ -225,11 +225,10 @@ static bool unit_steps(const dm::HostLayout &h, int d, int from, int step, int c
 
 // Both sides share their fastest axis d but not its order along it (for
 // example a block axis against an interleaved one): the gather would walk
-// 1D runs with scattered stores.  Split d into (z, y) with the source
-// contiguous along y and the destination along z, so the reshape becomes a
-// 2D tile transpose over (z, y).  The period P is the largest divisor of
-// the axis, up to sqrt of its length, that separates both sides that way
-// (or the smallest above it).
+// 1D runs with scattered stores.  Split d into (z, y) with one side
+// contiguous along y and the other along z, so the reshape becomes a 2D
+// tile transpose over (z, y).  Among the periods P that separate both sides
+// that way, the one with the squarest (z, y) is taken.
 static bool split_shared_axis(dm_reshape_program *pg) {
   dm::HostLayout &S = pg->src, &D = pg->dst;
   const int d = S.fastest;
@@ -237,17 +236,24 @@ static bool split_shared_axis(dm_reshape_program *pg) {
   const int n = S.L[d];
   if (n < 4 || (unit_steps(S, d, 0, 1, n) && unit_steps(D, d, 0, 1, n))) return false;
   int best = 0;
+  bool src_y = true;
   for (int P = 2; P <= n / 2; P++) {
     if (n % P != 0 || !separable(S, d, P) || !separable(D, d, P)) continue;
-    // source in runs along y, destination in runs along z
-    if (!unit_steps(S, d, 0, 1, P) || !unit_steps(D, d, 0, P, n / P)) continue;
-    if (best == 0 || std::min(P, n / P) > std::min(best, n / best)) best = P;
+    // one side in runs along y, the other in runs along z
+    const bool sy = unit_steps(S, d, 0, 1, P) && unit_steps(D, d, 0, P, n / P);
+    const bool sz = unit_steps(D, d, 0, 1, P) && unit_steps(S, d, 0, P, n / P);
+    if (!sy && !sz) continue;
+    if (best == 0 || std::min(P, n / P) > std::min(best, n / best)) {
+      best = P;
+      src_y = sy;
+    }
   }
   if (best == 0) return false;
   split_axis(S, d, best);
   split_axis(D, d, best);
-  S.fastest = S.ndim - 1;  // y
-  D.fastest = d;           // z
+  const int y = S.ndim - 1;
+  S.fastest = src_y ? y : d;
+  D.fastest = src_y ? d : y;
   return true;
 }
 

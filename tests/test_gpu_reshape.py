@@ -157,6 +157,10 @@ SAME_FASTEST = [
      [("block", 5, 3), ("local", 7), ("block", 36, 2)]),
     (4, [("local", 2), ("block", 10, 2), ("block", 50, 2)],
      [("local", 2), ("block", 10, 2), ("interleave", ("block", 10, 2), 5)]),
+    (4, [("block", 8, 2), ("local", 6), ("interleave", ("block", 4, 2), 8)],
+     [("block", 8, 2), ("local", 6), ("block", 32, 2)]),
+    (8, [("local", 4), ("block", 12, 2), ("interleave", ("block", 8, 4), 6)],
+     [("local", 4), ("block", 12, 2), ("block", 48, 4)]),
 ]
 
 
