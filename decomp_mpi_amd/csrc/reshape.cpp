@@ -511,9 +511,14 @@ static void run_reshape(MPI_Comm comm, ReshapePlan plan, void *src, void *dest) 
 
 // reshape.c:523-567, for every rank's messages; returns 1 on the first
 // violation after printing the reference's message.
+// reshape.c checks the calling process's own message lists.  Inside a
+// dm_spmd_run session each rank passes its own lengths, so only the view's
+// rank is checked; outside one, the call stands for every rank and each
+// rank's messages are checked against the same lengths.
 extern "C" int checkReshapePlanBounds(ReshapePlan plan, int type_size, int src_len, int dest_len) {
   int P = plan->comm->world->nranks;
-  for (int me = 0; me < P; me++) {
+  const bool own = dm::spmd_rank() >= 0;
+  for (int me = own ? plan->comm->rank : 0; me < (own ? plan->comm->rank + 1 : P); me++) {
     TypeEntryList sends = message_list(plan->src, plan->dest, plan->order, &dm_type_byte, me, true);
     TypeEntryList recvs = message_list(plan->dest, plan->src, plan->order, &dm_type_byte, me, false);
     // message_list with a 1-byte base counts elements; scale by type_size

@@ -156,8 +156,10 @@ void freeReshapePlan(ReshapePlan plan);
  *  launch; returns when the data has landed (unless the world is in
  *  asynchronous mode, see dm_set_async). */
 void runReshapePlan(MPI_Comm comm, ReshapePlan plan, void *src, void *dest);
-/** 0 when every rank's messages stay inside arrays of the given lengths,
- *  else 1 after a message on stderr (reshape.h:293). */
+/** 0 when the messages stay inside arrays of the given lengths, else 1
+ *  after a message on stderr (reshape.h:293).  Inside a dm_spmd_run
+ *  session the calling rank's own messages are checked, as under MPI;
+ *  outside one, every rank's against the same lengths. */
 int checkReshapePlanBounds(ReshapePlan plan, int type_size, int src_len,
                            int dest_len);
 /** reshape.c:494-521: the calling rank's file view of d over the global

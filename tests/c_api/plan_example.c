@@ -30,8 +30,12 @@ int main(void) {
   o[0] = makeAxisDecomp(world, makeSingleDecomp(SIZE, 0, 1), 1, 1);
   o[1] = makeAxisDecomp(world, makeSingleDecomp(SIZE, 0, npe), SIZE, 1);
   Decomp orig = makeDecomp(2, o);
-  n[0] = makeAxisDecomp(world, makeRotatedDecomp(2, makeLocalDecomp(SIZE)), 1, 1);
-  n[1] = makeAxisDecomp(world, makeRotatedDecomp(1, makeBlockDecomp(SIZE, npe)), SIZE, 1);
+  /* makeRotatedDecomp keeps a clone of what it wraps (rotate_decomp.c:47) */
+  DimDecomp inner0 = makeLocalDecomp(SIZE), inner1 = makeBlockDecomp(SIZE, npe);
+  n[0] = makeAxisDecomp(world, makeRotatedDecomp(2, inner0), 1, 1);
+  n[1] = makeAxisDecomp(world, makeRotatedDecomp(1, inner1), SIZE, 1);
+  freeDimDecomp(inner0);
+  freeDimDecomp(inner1);
   Decomp dest = makeDecomp(2, n);
   ReshapePlan plan = makeReshapePlan(world, MPI_INT, orig, dest);
   if (plan == NULL) failures++;
@@ -64,6 +68,10 @@ int main(void) {
   free(work);
   free(grid);
   freeReshapePlan(plan);
+  /* each Decomp owns its axes and their DimDecomps */
+  freeDecomp(start);
+  freeDecomp(orig);
+  freeDecomp(dest);
   dm_world_free(world);
   printf("%s\n", failures ? "FAILED" : "OK");
   return failures != 0;
