@@ -22,9 +22,11 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              f"-I{INCLUDE}", f"-I{CSRC}"]
 
 
-# FFT pass kernel families: launcher name, real type, inverse, powers of two only
-FFT_FAMILIES = [(f"launch_{r[0]}_{d}_{k}", r, "true" if d == "inv" else "false", "true" if k == "p2" else "false")
-                for r in ("double", "float") for d in ("fwd", "inv") for k in ("p2", "mx")]
+# FFT pass kernel families: launcher name, real type, inverse, powers of two
+# only, direct-sum stages (primes above 13)
+FFT_FAMILIES = [(f"launch_{r[0]}_{d}_{k}", r, "true" if d == "inv" else "false", "true" if k == "p2" else "false",
+                 "true" if k == "any" else "false")
+                for r in ("double", "float") for d in ("fwd", "inv") for k in ("p2", "mx", "any")]
 
 
 def _run(cmd):
@@ -51,10 +53,10 @@ def build(verbose=False):
     for s in hip:
         if os.path.basename(s) == "fft_inst.hip":
             # one translation unit per kernel family (they build in parallel)
-            for name, real, inv, pow2 in FFT_FAMILIES:
+            for name, real, inv, pow2, anyp in FFT_FAMILIES:
                 o = os.path.join(OBJDIR, f"fft_inst_{name}.o")
                 defs = [f"-DDM_INST_NAME={name}", f"-DDM_INST_R={real}", f"-DDM_INST_INV={inv}",
-                        f"-DDM_INST_POW2={pow2}"]
+                        f"-DDM_INST_POW2={pow2}", f"-DDM_INST_ANY={anyp}"]
                 jobs.append([f"{ROCM}/bin/hipcc", *HIP_FLAGS, *defs, "-c", "-o", o, s])
             continue
         o = os.path.join(OBJDIR, os.path.basename(s) + ".o")

@@ -311,6 +311,60 @@ __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T
   __syncthreads();
 }
 
+// One Stockham stage of a prime radix p without a kernel of its own (a
+// runtime value above 13), as direct sums: each work item is one output q of
+// one butterfly, y_q = sum_r (x_r * stage twiddle_r) * w_p^(r q mod p),
+// reading the butterfly's p inputs from LDS (lanes of a wave share
+// butterflies, so most reads are broadcasts).  O(p) per point; outputs wait
+// in registers until every item of the tile has read its inputs.  roots:
+// w_p^m for m < p, in the table's twiddle form.  The p products are summed in
+// double in either precision, so that a complex-float sum of up to 8191
+// terms keeps the error of one product (a float sum would grow with p).
+template <typename R, bool INV, int CAP, int NTH, bool CT, typename TW, typename RT>
+__device__ __forceinline__ void stage_lds_any(cx<R> *buf, const StageInfo &si, int p, int T, int ls, TW tw,
+                                              RT roots) {
+  constexpr int K = (CAP + NTH - 1) / NTH;
+  const int nr = si.nr, ns = si.ns;
+  const int items = T * nr * p;  // T * N <= CAP
+  const unsigned long long m_p = 0x100000000ull / (unsigned)p + 1;
+  cx<R> y[K];
+  int dst[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int e = threadIdx.x + k * NTH;
+    dst[k] = -1;
+    if (e < items) {
+      int b = fdiv(e, m_p);
+      int q = e - b * p;
+      int l = fdiv(b, si.m_nr);
+      int j = b - l * nr;
+      int jn = fdiv(j, si.m_ns);
+      int jj = j - jn * ns;
+      int base = l * ls;
+      TW tws = tw + (ns - 1) + jj;
+      const cx<R> x0 = buf[base + lds_pad(j)];
+      double ax = x0.x, ay = x0.y;
+      int m = 0;
+      for (int r = 1; r < p; r++) {
+        cx<R> v = buf[base + lds_pad(j + r * nr)];
+        if (ns > 1) v = twmul<INV, CT>(v, tws, (r - 1) * ns);
+        m += q;
+        if (m >= p) m -= p;
+        const cx<R> t = twmul<INV, CT>(v, roots, m);
+        ax += (double)t.x;
+        ay += (double)t.y;
+      }
+      y[k] = cx<R>{(R)ax, (R)ay};
+      dst[k] = (l << 16) | (jn * ns * p + jj + q * ns);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (dst[k] >= 0) buf[(dst[k] >> 16) * ls + lds_pad(dst[k] & 0xffff)] = y[k];
+  __syncthreads();
+}
+
 // Global data accesses of a pass: N elements at offsets a[k] from base,
 // optionally with the non-temporal hint (data streamed once per launch).
 // The hint is uniform across the launch, so the branch is taken once.

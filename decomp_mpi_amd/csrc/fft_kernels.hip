@@ -101,14 +101,19 @@ int launch_fft_pass(const FftLaunch &p, void *stream) {
   if (P.p.tile_base < 0 || P.p.tile_base + P.p.nblocks > all) return (int)hipErrorInvalidValue;
   size_t lds = fft_lds_bytes(p);
   hipStream_t s = (hipStream_t)stream;
-  bool pow2 = true;
-  for (int i = 0; i < p.nstage; i++) pow2 = pow2 && (p.radix[i] & (p.radix[i] - 1)) == 0;
+  bool pow2 = true, any = false;
+  for (int i = 0; i < p.nstage; i++) {
+    pow2 = pow2 && (p.radix[i] & (p.radix[i] - 1)) == 0;
+    any = any || !has_radix_kernel(p.radix[i]);
+  }
   hipError_t e;
   if (p.dbl) {
-    if (p.inverse) e = pow2 ? launch_d_inv_p2(P, lds, s) : launch_d_inv_mx(P, lds, s);
+    if (any) e = p.inverse ? launch_d_inv_any(P, lds, s) : launch_d_fwd_any(P, lds, s);
+    else if (p.inverse) e = pow2 ? launch_d_inv_p2(P, lds, s) : launch_d_inv_mx(P, lds, s);
     else e = pow2 ? launch_d_fwd_p2(P, lds, s) : launch_d_fwd_mx(P, lds, s);
   } else {
-    if (p.inverse) e = pow2 ? launch_f_inv_p2(P, lds, s) : launch_f_inv_mx(P, lds, s);
+    if (any) e = p.inverse ? launch_f_inv_any(P, lds, s) : launch_f_fwd_any(P, lds, s);
+    else if (p.inverse) e = pow2 ? launch_f_inv_p2(P, lds, s) : launch_f_inv_mx(P, lds, s);
     else e = pow2 ? launch_f_fwd_p2(P, lds, s) : launch_f_fwd_mx(P, lds, s);
   }
   return (int)e;

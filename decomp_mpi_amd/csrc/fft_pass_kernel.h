@@ -41,6 +41,10 @@ template <typename R, bool INV, int RADIX, int CAP, int NTH, bool CT, typename T
 __device__ __forceinline__ void stage(cx<R> *buf, const Pass &P, int s, TW tw) {
   stage_lds<R, INV, RADIX, CAP, NTH, CT>(buf, P.st[s], P.p.T, P.ls, tw);
 }
+template <typename R, bool INV, int CAP, int NTH, bool CT, typename TW, typename RT>
+__device__ __forceinline__ void stage_any(cx<R> *buf, const Pass &P, int s, TW tw, RT roots) {
+  stage_lds_any<R, INV, CAP, NTH, CT>(buf, P.st[s], P.p.radix[s], P.p.T, P.ls, tw, roots);
+}
 
 // Per-line address data of one side of a tile (in LDS).
 //   uniform:     base[l] = sum over axes other than t of abs[d][g_d]
@@ -278,7 +282,10 @@ __device__ __forceinline__ void last_stage_store(const cx<R> *buf, const Pass &P
 // Complex float at 8 points per thread is held to 64 VGPRs (8 waves per
 // SIMD: two 1024-thread or four 512-thread workgroups per CU).
 // CT: compensated twiddle products (complex float only).
-template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT>
+// ANY: the schedule has a prime radix above 13 (direct-sum stages); such
+// schedules get kernels of their own, so that the stage's registers do not
+// weigh on the others.
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT, bool ANY>
 __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) void fft_pass_kernel(Pass P) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
@@ -360,6 +367,10 @@ __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) vo
 
   // stages and the fused store, instantiated for LDS or global twiddles
   auto run_stages = [&](auto tw) {
+    // roots of the direct-sum stages: after the N stage entries of the
+    // device table, p per such stage in stage order (the first stage is
+    // never one when s0 = 1: the fused first stage is a power of two)
+    int roff = p.N;
     for (int s = s0; s < s1; s++) {
       switch (p.radix[s]) {
         case 2: stage<R, INV, 2, CAP, NTH, CT>(buf, P, s, tw); break;
@@ -371,7 +382,15 @@ __global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 1) vo
         case 11: if constexpr (!POW2) stage<R, INV, 11, CAP, NTH, CT>(buf, P, s, tw); break;
         case 13: if constexpr (!POW2) stage<R, INV, 13, CAP, NTH, CT>(buf, P, s, tw); break;
         case 16: stage<R, INV, 16, CAP, NTH, CT>(buf, P, s, tw); break;
-        default: break;
+        default:
+          if constexpr (ANY) {
+            if constexpr (sizeof(R) == 4 && !CT)
+              stage_any<R, INV, CAP, NTH, CT>(buf, P, s, tw, tw_hi{reinterpret_cast<const cx<R> *>(twg)} + roff);
+            else
+              stage_any<R, INV, CAP, NTH, CT>(buf, P, s, tw, twg + roff);
+            roff += p.radix[s];
+          }
+          break;
       }
     }
 
@@ -429,11 +448,11 @@ inline unsigned long long magic_for(int d) { return (0x100000000ULL / (unsigned 
 constexpr int CAP_D = 4096;  // complex double points per big tile
 constexpr int CAP_F = 8192;  // complex float points per big tile
 constexpr int NTH_BIG = 512;
-template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT>
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT, bool ANY>
 hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
   static bool attr_set = false;
   static int cus = 0;
-  auto k = fft_pass_kernel<R, INV, CAP, NTH, POW2, UIN, UOUT, CT>;
+  auto k = fft_pass_kernel<R, INV, CAP, NTH, POW2, UIN, UOUT, CT, ANY>;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
@@ -466,20 +485,20 @@ hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename R, bool INV, int CAP, int NTH, bool POW2, bool CT>
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool CT, bool ANY>
 hipError_t launch_layouts(const Pass &P, size_t lds, hipStream_t s) {
   bool ui = P.p.sin.uniform, uo = P.p.sout.uniform;
-  if (ui && uo) return launch_typed<R, INV, CAP, NTH, POW2, true, true, CT>(P, lds, s);
-  if (ui) return launch_typed<R, INV, CAP, NTH, POW2, true, false, CT>(P, lds, s);
-  if (uo) return launch_typed<R, INV, CAP, NTH, POW2, false, true, CT>(P, lds, s);
-  return launch_typed<R, INV, CAP, NTH, POW2, false, false, CT>(P, lds, s);
+  if (ui && uo) return launch_typed<R, INV, CAP, NTH, POW2, true, true, CT, ANY>(P, lds, s);
+  if (ui) return launch_typed<R, INV, CAP, NTH, POW2, true, false, CT, ANY>(P, lds, s);
+  if (uo) return launch_typed<R, INV, CAP, NTH, POW2, false, true, CT, ANY>(P, lds, s);
+  return launch_typed<R, INV, CAP, NTH, POW2, false, false, CT, ANY>(P, lds, s);
 }
 
-template <typename R, bool INV, int CAP, int NTH, bool POW2>
+template <typename R, bool INV, int CAP, int NTH, bool POW2, bool ANY>
 hipError_t launch_twiddles(const Pass &P, size_t lds, hipStream_t s) {
   if constexpr (sizeof(R) == 4)
-    if (P.p.ctw) return launch_layouts<R, INV, CAP, NTH, POW2, true>(P, lds, s);
-  return launch_layouts<R, INV, CAP, NTH, POW2, false>(P, lds, s);
+    if (P.p.ctw) return launch_layouts<R, INV, CAP, NTH, POW2, true, ANY>(P, lds, s);
+  return launch_layouts<R, INV, CAP, NTH, POW2, false, ANY>(P, lds, s);
 }
 
 // Power-of-two radix schedules get a kernel without the prime butterflies
@@ -488,21 +507,21 @@ hipError_t launch_twiddles(const Pass &P, size_t lds, hipStream_t s) {
 // (half the LDS, twice the workgroups per CU), or the wide tile.  Complex
 // float: CAP points on 1024 threads or CAP/2 on 512 (at 16 points per
 // thread, 512 threads for CAP measured slower).
-template <typename R, bool INV, bool POW2>
+template <typename R, bool INV, bool POW2, bool ANY>
 hipError_t launch_tiles(const Pass &P, size_t lds, hipStream_t s) {
   constexpr int CAP = sizeof(R) == 8 ? CAP_D : CAP_F;
   const bool small = P.p.T * P.p.N <= CAP / 2;
   if constexpr (sizeof(R) == 4) {
-    if (small) return launch_twiddles<R, INV, CAP / 2, NTH_BIG, POW2>(P, lds, s);
-    return launch_twiddles<R, INV, CAP, 2 * NTH_BIG, POW2>(P, lds, s);
+    if (small) return launch_twiddles<R, INV, CAP / 2, NTH_BIG, POW2, ANY>(P, lds, s);
+    return launch_twiddles<R, INV, CAP, 2 * NTH_BIG, POW2, ANY>(P, lds, s);
   } else {
     // wide complex-double tile: 16 lines (256-byte column rows) on 1024
     // threads, for column walks at a large line stride (for complex float
     // the same 256-byte rows measured slower: 2.66 vs 2.58 ms per 512^3
     // step)
-    if (P.p.T * P.p.N > CAP) return launch_twiddles<R, INV, 2 * CAP, 2 * NTH_BIG, POW2>(P, lds, s);
-    if (small) return launch_twiddles<R, INV, CAP / 2, NTH_BIG / 2, POW2>(P, lds, s);
-    return launch_twiddles<R, INV, CAP, NTH_BIG, POW2>(P, lds, s);
+    if (P.p.T * P.p.N > CAP) return launch_twiddles<R, INV, 2 * CAP, 2 * NTH_BIG, POW2, ANY>(P, lds, s);
+    if (small) return launch_twiddles<R, INV, CAP / 2, NTH_BIG / 2, POW2, ANY>(P, lds, s);
+    return launch_twiddles<R, INV, CAP, NTH_BIG, POW2, ANY>(P, lds, s);
   }
 }
 
@@ -516,6 +535,10 @@ hipError_t launch_f_fwd_p2(const Pass &P, size_t lds, hipStream_t s);
 hipError_t launch_f_fwd_mx(const Pass &P, size_t lds, hipStream_t s);
 hipError_t launch_f_inv_p2(const Pass &P, size_t lds, hipStream_t s);
 hipError_t launch_f_inv_mx(const Pass &P, size_t lds, hipStream_t s);
+hipError_t launch_d_fwd_any(const Pass &P, size_t lds, hipStream_t s);
+hipError_t launch_d_inv_any(const Pass &P, size_t lds, hipStream_t s);
+hipError_t launch_f_fwd_any(const Pass &P, size_t lds, hipStream_t s);
+hipError_t launch_f_inv_any(const Pass &P, size_t lds, hipStream_t s);
 
 }  // namespace fftpass
 }  // namespace dm

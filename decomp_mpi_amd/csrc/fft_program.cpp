@@ -28,11 +28,13 @@ Side make_side(const HostLayout &h, i64 slot) {
 }
 
 // Odd prime factors one stage each; powers of two in radix-8 stages with one
-// 16/4/2 stage for the remainder.
-bool choose_radices(int n, std::vector<int> *out) {
+// 16/4/2 stage for the remainder.  any_prime: primes above 13 become stages
+// of the direct-sum kernel, last in the schedule (the pass kernel's
+// general-radix stage); without it such a length is refused.
+bool choose_radices(int n, std::vector<int> *out, bool any_prime) {
   out->clear();
   if (n < 1) return false;
-  std::vector<int> odd;
+  std::vector<int> odd, big;
   const int primes[] = {13, 11, 7, 5, 3};
   for (int p : primes)
     while (n % p == 0) {
@@ -44,6 +46,12 @@ bool choose_radices(int n, std::vector<int> *out) {
     n /= 2;
     k++;
   }
+  if (any_prime)
+    for (int p = 17; n > 1 && p <= n; p += 2)
+      while (n % p == 0) {
+        big.push_back(p);
+        n /= p;
+      }
   if (n != 1) return false;
   std::vector<int> pw;
   while (k >= 3) {
@@ -58,6 +66,7 @@ bool choose_radices(int n, std::vector<int> *out) {
   }
   out->insert(out->end(), pw.begin(), pw.end());
   out->insert(out->end(), odd.begin(), odd.end());
+  out->insert(out->end(), big.begin(), big.end());
   return true;
 }
 
@@ -114,8 +123,8 @@ bool axis_supported(int n, int dbl, std::string *why) {
   if (n > cap) {
     snprintf(msg, sizeof msg, "axis length %d exceeds the %d-point tile of the %s pass", n, cap,
              dbl ? "complex-double" : "complex-float");
-  } else if (!choose_radices(n, &rad) || (int)rad.size() > 16) {
-    snprintf(msg, sizeof msg, "axis length %d has a prime factor above 13", n);
+  } else if (!choose_radices(n, &rad, true) || (int)rad.size() > 16) {
+    snprintf(msg, sizeof msg, "axis length %d has more than 16 prime factors", n);
   } else {
     return true;
   }
@@ -151,8 +160,8 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     return false;
   }
   std::vector<int> rad;
-  if (!choose_radices(L.N, &rad) || (int)rad.size() > 16) {
-    raise_error(comm, "FFT: axis length %d has a prime factor above 13", L.N);
+  if (!choose_radices(L.N, &rad, true) || (int)rad.size() > 16) {
+    raise_error(comm, "FFT: axis length %d has more than 16 prime factors", L.N);
     return false;
   }
   L.nstage = (int)rad.size();
@@ -352,9 +361,21 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
         }
       ns *= R;
     }
+    // after the N stage entries: w_p^m = w^(m N / p), m < p, for each stage
+    // of a radix without a kernel of its own, in stage order
+    for (int st = 0; st < L.nstage; st++) {
+      int R = L.radix[st];
+      if (has_radix_kernel(R)) continue;
+      for (int m = 0; m < R; m++) {
+        size_t k = (size_t)m * (L.N / R);
+        w.push_back(root[2 * k]);
+        w.push_back(root[2 * k + 1]);
+      }
+    }
   }
-  std::vector<char> img((size_t)L.N * 16);
-  twiddle_image(w.data(), (size_t)L.N, dbl, img.data());
+  const size_t ntw = w.size() / 2;
+  std::vector<char> img(ntw * 16);
+  twiddle_image(w.data(), ntw, dbl, img.data());
   pp->tw = (long)tb.put(img.data(), img.size());
   L.in = in;
   L.out = out;

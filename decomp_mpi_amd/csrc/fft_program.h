@@ -18,9 +18,10 @@ struct Side {
 };
 Side make_side(const HostLayout &h, i64 slot);
 
-bool choose_radices(int n, std::vector<int> *out);
+bool choose_radices(int n, std::vector<int> *out, bool any_prime = false);
 // Whether the pass kernel transforms lines of n points in this precision
-// (n within one tile, radices 2..16, 3, 5, 7, 11, 13); why: the reason if not.
+// (n within one tile: radices 2..16, 3, 5, 7, 11, 13 by their own stage
+// kernels, larger primes by direct sums); why: the reason if not.
 bool axis_supported(int n, int dbl, std::string *why);
 // exp(-2 pi i k / n), k < n, interleaved (re, im), accurate to double rounding
 std::vector<double> roots(int n);

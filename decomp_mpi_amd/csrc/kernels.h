@@ -95,6 +95,11 @@ struct FftLaunch {
   int stream_in, stream_out;  // 1: non-temporal hint on the data loads / stores
   int ctw;                    // complex float: 1 compensated twiddle products, 0 the float part only
 };
+// Radices with a stage kernel of their own; any other (a prime above 13)
+// runs as a direct p-point sum.
+inline bool has_radix_kernel(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13 || r == 16;
+}
 // Tiles of a whole pass: tiles along u times the batch, the merged axis
 // counting in blocks of merge_k.
 inline long long fft_pass_tiles(const FftLaunch &p) {

@@ -155,6 +155,11 @@ def owned_mask(d, width):
     (8, (16, 16, 16), (2, 2, 2), [1, 4, 1]),
     (8, (16, 16, 16), (1, 2, 4), [1, 1, 16]),     # the 512^3 bench family, scaled down
     (32, (64, 64, 64), (1, 4, 8), [1, 1, 32]),
+    (4, (34, 38, 16), (1, 2, 2), [2, 2, 4]),      # primes 17, 19: direct-sum stages after a radix-2 one
+    (4, (17, 23, 16), (1, 2, 2), None),           # prime axes, single direct-sum stage
+    (2, (289, 6, 4), (2, 1, 1), [17, 2, 4]),      # 17 x 17: a direct-sum stage with stage twiddles
+    (2, (254, 4, 4), (2, 1, 1), [2, 4, 4]),       # 2 x 127
+    (2, (4093, 2, 2), (2, 1, 1), None),           # the largest prime within one complex-double tile
 ])
 def test_split_fft_fp64_vs_oracle(C, P, shape, grid, B):
     if B is None:
@@ -176,6 +181,9 @@ def test_split_fft_slab(C):
     (8, (16, 16, 16), (2, 2, 2), [1, 4, 1]),
     (8, (16, 16, 16), (1, 2, 4), [1, 1, 16]),
     (4, (22, 26, 16), (1, 2, 2), [2, 2, 4]),      # radices 11, 13
+    (4, (34, 38, 16), (1, 2, 2), [2, 2, 4]),      # primes 17, 19 (direct sums)
+    (2, (289, 6, 4), (2, 1, 1), [17, 2, 4]),
+    (2, (8191, 2, 2), (2, 1, 1), None),           # the largest prime within one complex-float tile
 ])
 def test_split_fft_fp32_vs_fp64_oracle(C, P, shape, grid, B):
     if B is None:
@@ -211,6 +219,7 @@ PHASED = [
     (8, (16, 16, 16), (2, 2, 2), [1, 4, 1], None),     # B_d = 1 beside B_d > 1
     (2, (8, 8, 8), (1, 1, 2), [1, 1, 1], 4096),        # single second stage (B = 1)
     (4, (16, 16), (2, 2), [4, 4], None),               # 2D
+    (4, (34, 38, 16), (1, 2, 2), [2, 2, 4], None),     # 17- and 19-point direct-sum second stages
 ]
 
 

@@ -168,8 +168,10 @@ def test_visit_data_reports_every_slot(C):
 
 
 @pytest.mark.parametrize("L,dbl,A,ok,phased", [
-    (272, 1, 16, False, None),       # 17 * 16: a prime factor above 13
-    (17 * 8, 0, 8, False, None),
+    (272, 1, 16, True, False),       # 17 * 16: a prime above 13 runs as direct sums
+    (17 * 8, 0, 8, True, False),
+    (4099, 1, 1, False, None),       # a prime longer than a complex-double tile
+    (8209, 0, 1, False, None),       # a prime longer than a complex-float tile
     (8192, 1, 1, False, None),       # longer than a pass tile, and so is B = 8192
     (8192, 1, 64, True, True),       # 64 x 128: phase order only
     (65536, 1, 256, True, True),
@@ -180,7 +182,7 @@ def test_visit_data_reports_every_slot(C):
 ])
 def test_split_plan_axis_support_at_plan_time(C, L, dbl, A, ok, phased):
     """An axis that neither one fused pass nor the plan's A x B factors can
-    transform (a prime factor above 13, or a factor longer than one tile)
+    transform (a factor longer than one tile, a prime one included)
     makes buildParallelPlan return NULL, as the reference's other plan
     rejections do (split_fft.c:252-308), and makeSplitPlan raise 'Cannot
     generate plan' with the reason.  An axis longer than one pass tile
