@@ -288,12 +288,10 @@ template <typename R, bool INV, int CAP, bool PRIMES, bool TWL>
 hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
   constexpr int NTH = BNTH;
   auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL>;
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
+  // once per kernel (a thread-safe static)
+  static const hipError_t attr =
+      hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) return attr;
   hipLaunchKernelGGL(k, dim3((unsigned)p.grid), dim3(NTH), p.lds, s, p);
   return hipGetLastError();
 }

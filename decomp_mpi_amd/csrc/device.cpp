@@ -19,12 +19,11 @@ static void hip_check(hipError_t e, const char *what) {
 }
 
 bool device_available() {
-  static int cached = -1;
-  if (cached < 0) {
+  static const bool avail = [] {
     int n = 0;
-    cached = (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
-  }
-  return cached == 1;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+  }();
+  return avail;
 }
 
 void check_device(dm_comm_t comm) {

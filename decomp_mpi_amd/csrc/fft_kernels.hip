@@ -20,11 +20,10 @@ namespace dm {
 // workgroups per CU).  DM_FFT_TILE=big|small forces one size.
 int fft_tile_capacity(int dbl, int column, int wide) {
   using namespace fftpass;
-  static int force = -2;
-  if (force == -2) {
+  static const int force = [] {
     const char *e = getenv("DM_FFT_TILE");
-    force = e == nullptr ? -1 : (e[0] == 'b' ? 0 : 1);
-  }
+    return e == nullptr ? -1 : (e[0] == 'b' ? 0 : 1);
+  }();
   int cap = dbl ? CAP_D : CAP_F;
   if (dbl && column && wide && force < 0) return 2 * cap;
   bool small = force >= 0 ? force == 1 : !column;
@@ -68,8 +67,7 @@ int launch_fft_pass(const FftLaunch &p, void *stream) {
     int r0 = p.radix[0], rl = p.radix[p.nstage > 0 ? p.nstage - 1 : 0];
     P.fuse_in = p.nstage >= 1 && pow2(r0) && V % r0 == 0 && (!p.in_row || p.ord_in == nullptr);
     P.fuse_out = p.nstage >= 2 && pow2(rl) && V % rl == 0 && (!p.out_row || p.ord_out == nullptr);
-    static int off = -1;
-    if (off < 0) off = getenv("DM_NO_FUSE") != nullptr;
+    static const bool off = getenv("DM_NO_FUSE") != nullptr;
     if (off) P.fuse_in = P.fuse_out = 0;
     // compensated twiddles with per-rank output strides: the fused store's
     // address arithmetic and the twiddle pairs do not fit 64 VGPRs, so the

@@ -450,37 +450,37 @@ constexpr int CAP_F = 8192;  // complex float points per big tile
 constexpr int NTH_BIG = 512;
 template <typename R, bool INV, int CAP, int NTH, bool POW2, bool UIN, bool UOUT, bool CT, bool ANY>
 hipError_t launch_typed(const Pass &P, size_t lds, hipStream_t s) {
-  static bool attr_set = false;
-  static int cus = 0;
   auto k = fft_pass_kernel<R, INV, CAP, NTH, POW2, UIN, UOUT, CT, ANY>;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
+  // Once per kernel (a thread-safe static): the LDS limit, and the CU count
+  // and VGPR use for the residency report.
+  struct Once {
+    hipError_t attr;
+    int cus, vgprs;
+  };
+  static const Once once = [k] {
+    Once o{hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024), 0, 128};
     int dev = 0;
     (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    attr_set = true;
-  }
-  // One workgroup per tile.  Residency, reported under DM_DEBUG, is
-  // computed here from LDS and VGPR use (512 VGPRs per SIMD lane, at most 8
-  // waves per SIMD, 4 SIMDs per CU): the runtime's occupancy query assumes
-  // 64 KiB of LDS per CU, and gfx950 has 160 KiB.
-  static int vgprs = -1;
-  if (vgprs < 0) {
+    (void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipFuncAttributes fa;
-    vgprs = (hipFuncGetAttributes(&fa, (const void *)k) == hipSuccess && fa.numRegs > 0) ? fa.numRegs : 128;
-  }
-  int waves_per_wg = NTH / 64;
-  int lds_fit = (int)((160 * 1024) / std::max<size_t>(lds, 1));
-  int waves_per_simd = std::min(8, 512 / (((vgprs + 7) / 8) * 8));
-  int vgpr_fit = (waves_per_simd * 4) / waves_per_wg;
-  int per_cu = std::max(1, std::min(lds_fit, vgpr_fit));
-  long long grid = P.p.nblocks;  // one tile per workgroup
-  static int debug = -1;
-  if (debug < 0) debug = getenv("DM_DEBUG") != nullptr;
-  if (debug)
+    if (hipFuncGetAttributes(&fa, (const void *)k) == hipSuccess && fa.numRegs > 0) o.vgprs = fa.numRegs;
+    return o;
+  }();
+  if (once.attr != hipSuccess) return once.attr;
+  const long long grid = P.p.nblocks;  // one tile per workgroup
+  static const bool debug = getenv("DM_DEBUG") != nullptr;
+  if (debug) {
+    // Residency from LDS and VGPR use (512 VGPRs per SIMD lane, at most 8
+    // waves per SIMD, 4 SIMDs per CU): the runtime's occupancy query assumes
+    // 64 KiB of LDS per CU, and gfx950 has 160 KiB.
+    int waves_per_wg = NTH / 64;
+    int lds_fit = (int)((160 * 1024) / std::max<size_t>(lds, 1));
+    int waves_per_simd = std::min(8, 512 / (((once.vgprs + 7) / 8) * 8));
+    int vgpr_fit = (waves_per_simd * 4) / waves_per_wg;
+    int per_cu = std::max(1, std::min(lds_fit, vgpr_fit));
     fprintf(stderr, "fft pass N=%d T=%d tiles=%lld grid=%lld per_cu=%d cus=%d lds=%zu\n", P.p.N, P.p.T,
-            P.p.nblocks, grid, per_cu, cus, lds);
+            P.p.nblocks, grid, per_cu, once.cus, lds);
+  }
   hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(NTH), lds, s, P);
   return hipGetLastError();
 }

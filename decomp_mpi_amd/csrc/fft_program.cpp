@@ -186,8 +186,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     const size_t es = dbl ? 16 : 8;
     bool wide = (!in_row && (size_t)I.h.m[t] * es >= (64u << 10)) ||
                 (!out_row && (size_t)U.h.m[t] * es >= (64u << 10));
-    static int nowide = -1;
-    if (nowide < 0) nowide = getenv("DM_NO_WIDE") != nullptr;
+    static const bool nowide = getenv("DM_NO_WIDE") != nullptr;
     int tcap = fft_tile_capacity(dbl, !(in_row && out_row), wide && !nowide);
     L.T = std::max(1, std::min(std::min(tcap / L.N, I.h.L[u]), 512));
     L.tiles_u = (I.h.L[u] + L.T - 1) / L.T;
@@ -203,8 +202,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     // (complex double: no hint at all there, 4.63 ms against 4.85 with the
     // hint on its loads)
     const size_t es = dbl ? 16 : 8;
-    static int nont = -1;
-    if (nont < 0) nont = getenv("DM_NO_NONTEMPORAL") != nullptr;
+    static const bool nont = getenv("DM_NO_NONTEMPORAL") != nullptr;
     const bool far = !out_row && (size_t)U.h.m[t] * es >= (64u << 10);
     L.stream_in = (nont || (far && dbl)) ? 0 : 1;
     L.stream_out = (nont || far) ? 0 : 1;
@@ -261,8 +259,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
     if (s.h.uniform && s.h.m[t] > 0) {
       bool affine = true;
       for (int x = 0; x < s.h.L[t] && affine; x++) affine = s.h.rank_part[t][x] == 0 && s.h.off[t][x] == x;
-      static int noaff = -1;
-      if (noaff < 0) noaff = getenv("DM_NO_AFFINE") != nullptr;
+      static const bool noaff = getenv("DM_NO_AFFINE") != nullptr;
       if (affine && !noaff) fs.aff = s.h.m[t];
       const std::vector<i64> &a = s.abs[t];
       const i64 m = s.h.m[t];

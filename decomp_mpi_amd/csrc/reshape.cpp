@@ -473,8 +473,7 @@ bool enqueue_reshape(dm_comm_t comm, ReshapePlan plan, const void *src, void *ds
   L.elem_bytes = pg->word;
   // non-temporal hint: 11-20% off the 512^3 int64 hops (each element is
   // read once and written once)
-  static int nont = -1;
-  if (nont < 0) nont = getenv("DM_NO_NONTEMPORAL") != nullptr;
+  static const bool nont = getenv("DM_NO_NONTEMPORAL") != nullptr;
   L.nontemporal = nont ? 0 : 1;
   return launch_ok(comm, launch_reshape(L, stream), "reshape kernel launch");
 }

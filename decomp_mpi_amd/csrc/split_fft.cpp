@@ -322,11 +322,10 @@ static bool compile_direction(SplitPlan plan, bool inverse, const dm::Side &S, c
   for (int d = 0; d < nd; d++)
     if (x->L[d] > 1) axes.push_back(d);
   if (axes.empty()) axes.push_back(0);
-  static int dense_env = -1;
-  if (dense_env < 0) {
+  static const int dense_env = [] {
     const char *e = getenv("DM_DENSE_INTERMEDIATES");
-    dense_env = e ? atoi(e) : 1;
-  }
+    return e ? atoi(e) : 1;
+  }();
   const int n = (int)axes.size();
   bool dense = dense_env != 0 && n >= 2;
   // pass order: dense -- the first-read side's fastest axis, then the rest

@@ -179,9 +179,8 @@ extern "C" void raise_error(MPI_Comm comm, const char *msg, ...) {
 
 // ---------------------------------------------------------------- logging
 
-static int debug_enabled() {
-  static int v = -1;
-  if (v < 0) v = getenv("DM_DEBUG") != nullptr;
+static bool debug_enabled() {
+  static const bool v = getenv("DM_DEBUG") != nullptr;
   return v;
 }
 
@@ -203,14 +202,16 @@ extern "C" void fragment_logger(const char *msg, ...) {
 }
 
 extern "C" double timer(void) {
-  static bool called = false;
+  // the first call (from any thread) sets the base and returns 0
+  static std::once_flag once;
   static struct timespec base;
-  struct timespec now;
-  if (!called) {
-    called = true;
+  bool first = false;
+  std::call_once(once, [&] {
     clock_gettime(CLOCK_MONOTONIC, &base);
-    return 0.0;
-  }
+    first = true;
+  });
+  if (first) return 0.0;
+  struct timespec now;
   clock_gettime(CLOCK_MONOTONIC, &now);
   return (double)(now.tv_sec - base.tv_sec) + 1e-9 * (double)(now.tv_nsec - base.tv_nsec);
 }

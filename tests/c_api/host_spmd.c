@@ -1,5 +1,6 @@
 /* The SPMD layer's host-side synchronisation, written for a build of the C
  * layer with the thread sanitizer: P threads (one per virtual rank) call
+ * timer() (whose first call from any thread sets the base) and
  * the all-reduces and the barrier many times in a row, each builds and
  * frees its own decompositions and reshape plans between them, and in one
  * session a collective reshape whose ranks pass different destination
@@ -15,6 +16,7 @@
 #include "dm_spmd.h"
 #include "local_decomp.h"
 #include "reshape.h"
+#include "timer.h"
 
 #define P 6
 #define ROUNDS 2000
@@ -27,6 +29,7 @@ static void body(MPI_Comm me, void *arg) {
   (void)arg;
   int r;
   MPI_Comm_rank(me, &r);
+  const double t0 = timer();
   for (int i = 0; i < ROUNDS; i++) {
     long long s = dm_spmd_allreduce_sum(me, r + i);
     if (s != (long long)P * (P - 1) / 2 + (long long)P * i) bad[r]++;
@@ -45,6 +48,7 @@ static void body(MPI_Comm me, void *arg) {
     }
     dm_spmd_barrier(me);
   }
+  if (!(timer() >= t0 && t0 >= 0.0)) bad[r]++;
   /* rank 1 passes its destination 16 bytes further in than the others */
   DimDecomp dd[1] = {makeBlockDecomp(12, P)};
   Decomp d = makePackedDecomp(me, 1, dd);
