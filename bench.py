@@ -307,7 +307,8 @@ def batched_bench(C, dm, steps, warmup):
             "GFLOP_per_s": round(flops / dt / 1e9, 1), "GB_per_s": round(alg / dt / 1e9, 1),
             "frac_of_peak": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
             "max_rel_l2_vs_fp64": max_rel, "max_abs_err_over_max_vs_fp64": max_abs,
-            "schedule": {"tile": ["full", "half"][tile.value], "radix_order": ["large_first", "small_first"][order.value],
+            "schedule": {"tile": ["full", "half", "planes (last two axes per plane tile where they fit)"][tile.value],
+                         "radix_order": ["large_first", "small_first"][order.value],
                          "grid": ["one workgroup per tile", "persistent"][grid_mode],
                          "autotune_best_ms": round(tune_ms, 4)}}
 

@@ -31,6 +31,7 @@ FFTW_FORWARD = -1
 FFTW_BACKWARD = 1
 NO_FLAGS, PREFER_LARGE_FFT, PREFER_STAGE1, FORCE_USE_FIRST, RUN_PHASE_ORDER = 0, 1, 2, 4, 0x100
 DM_TWIDDLES_COMPENSATED, DM_TWIDDLES_PLAIN = 0, 1
+DM_BATCH_TILE_FULL, DM_BATCH_TILE_HALF, DM_BATCH_TILE_PLANES = 0, 1, 2
 RESHAPE_1, FFT_1, TWIDDLE, RESHAPE_2, FFT_2, ALL, NUM_TIMERS = range(7)
 
 

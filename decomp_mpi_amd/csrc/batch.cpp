@@ -52,6 +52,11 @@ struct dm_batch {
   int primes = 0;  // some schedule has a radix 5, 7, 11 or 13
   int long_lines = 0;  // some schedule's twiddles stay in device memory
   std::vector<PassTables> passes;
+  // plane tiles (DM_BATCH_TILE_PLANES), run first: [0] within the line
+  // tile's points, [1] larger
+  PassTables planes[2];
+  size_t plane_lds[2] = {0, 0};
+  int plane_primes[2] = {0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void *graph_stream = nullptr;
@@ -69,6 +74,10 @@ static void release_bound(DmBatch b) {
   drop_graph(b);
   for (auto &p : b->passes) dm::device_free(p.dev);
   b->passes.clear();
+  for (auto &pl : b->planes) {
+    dm::device_free(pl.dev);
+    pl = PassTables();
+  }
   dm::device_free(b->d_ptrs);
   dm::device_free(b->d_sched);
   dm::device_free(b->d_tw);
@@ -156,13 +165,20 @@ void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vecto
 
 // Host image of a batch: tiles per axis pass (last axis first), schedules
 // and their stage-ordered twiddles (complex double), LDS need, and whether a
-// radix 5, 7, 11 or 13 occurs.
+// radix 5, 7, 11 or 13 occurs.  DM_BATCH_TILE_PLANES: a transform whose
+// last two axes make a plane the plane kernel holds gets plane tiles (one
+// per plane) instead of line tiles for those two axes.
 bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_choice, int radix_order,
                       BatchHost *out) {
   BatchHost &h = *out;
   h = BatchHost();
   const int cap = batch_tile_capacity(dbl);
   std::map<std::pair<int, int>, int> sched_of;  // (N, T) -> schedule
+  auto has_primes = [](const BatchSched &S) {
+    for (int s = 0; s < S.nstage; s++)
+      if (S.radix[s] == 5 || S.radix[s] == 7 || S.radix[s] == 11 || S.radix[s] == 13) return true;
+    return false;
+  };
   auto schedule = [&](int N, int T) -> int {
     auto key = std::make_pair(N, T);
     auto it = sched_of.find(key);
@@ -171,20 +187,49 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
     std::vector<double> block;
     make_batch_sched(N, T, radix_order, &S, &block);
     S.tw_off = (int)(h.tw.size() / 2);
-    for (int s = 0; s < S.nstage; s++)
-      if (S.radix[s] == 5 || S.radix[s] == 7 || S.radix[s] == 11 || S.radix[s] == 13) h.primes = 1;
-    if (!batch_tw_in_lds(N)) h.long_lines = 1;
     h.tw.insert(h.tw.end(), block.begin(), block.end());
     int id = (int)h.sched.size();
     h.sched.push_back(S);
     sched_of[key] = id;
+    return id;
+  };
+  // a schedule used by line tiles
+  auto line_schedule = [&](int N, int T) -> int {
+    int id = schedule(N, T);
+    if (has_primes(h.sched[id])) h.primes = 1;
+    if (!batch_tw_in_lds(N)) h.long_lines = 1;
     h.lds = std::max(h.lds, batch_lds_bytes(dbl, T, N));
     return id;
   };
   h.tiles.assign(ndim, std::vector<BatchTile>());
   for (int i = 0; i < count; i++) {
     const int *n = &shapes[(size_t)i * ndim];
-    for (int d = 0; d < ndim; d++) {
+    int dlast = ndim;  // axes [0, dlast) get line tiles
+    if (tile_choice == DM_BATCH_TILE_PLANES && ndim >= 2) {
+      const int n1 = n[ndim - 2], n2 = n[ndim - 1];
+      const size_t plds = batch_plane_lds_bytes(dbl, n1, n2);
+      if (plds > 0 && n1 <= cap && n2 <= cap) {
+        long long outer = 1;
+        for (int e = 0; e < ndim - 2; e++) outer *= n[e];
+        const int rows = schedule(n2, n1), cols = schedule(n1, n2);
+        const int k = (long long)n1 * n2 <= cap ? 0 : 1;
+        if (has_primes(h.sched[rows]) || has_primes(h.sched[cols])) h.plane_primes[k] = 1;
+        h.plane_lds[k] = std::max(h.plane_lds[k], plds);
+        for (long long o = 0; o < outer; o++) {
+          BatchTile t{};
+          t.base = o * n1 * n2;
+          t.ps = 1;
+          t.xform = i;
+          t.sched = rows;
+          t.sched2 = cols;
+          t.nl = n1;
+          t.row = 2;
+          h.planes[k].push_back(t);
+        }
+        dlast = ndim - 2;
+      }
+    }
+    for (int d = 0; d < dlast; d++) {
       long long inner = 1, outer = 1;
       for (int e = d + 1; e < ndim; e++) inner *= n[e];
       for (int e = 0; e < d; e++) outer *= n[e];
@@ -194,7 +239,7 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
       // lines per tile: as many as the tile holds; a column walk never more
       // than the contiguous run (so no tile is mostly padding)
       const int T = batch_tile_height(N, inner, dbl, tile_choice);
-      int sid = schedule(N, T);
+      int sid = line_schedule(N, T);
       auto &tl = h.tiles[ndim - 1 - d];  // passes run last axis first
       if (row) {
         for (long long o = 0; o < outer; o += T) {
@@ -239,6 +284,11 @@ static bool bind_tables(DmBatch b) {
   b->long_lines = h.long_lines;
   b->passes.assign(b->ndim, PassTables());
   for (int d = 0; d < b->ndim; d++) b->passes[d].tiles = std::move(h.tiles[d]);
+  for (int k = 0; k < 2; k++) {
+    b->planes[k].tiles = std::move(h.planes[k]);
+    b->plane_lds[k] = h.plane_lds[k];
+    b->plane_primes[k] = h.plane_primes[k];
+  }
   std::vector<dm::BatchSched> &sched = h.sched;
   std::vector<double> &tw = h.tw;
   // upload
@@ -270,6 +320,19 @@ static bool bind_tables(DmBatch b) {
     p.grid = b->grid_mode == DM_BATCH_GRID_PERSISTENT ? std::min<long long>((long long)p.tiles.size(), resident)
                                                       : (long long)p.tiles.size();
   }
+  for (int k = 0; k < 2; k++) {
+    PassTables &pl = b->planes[k];
+    if (pl.tiles.empty()) continue;
+    pl.dev = up(pl.tiles.data(), pl.tiles.size() * sizeof(dm::BatchTile));
+    if (!pl.dev) {
+      raise_error(nullptr, "dm_batch_bind: device allocation or copy failed");
+      return false;
+    }
+    const long long res = k == 0 ? dm::batch_resident_groups(b->dbl, b->plane_primes[0], b->plane_lds[0])
+                                 : dm::batch_plane_resident_groups(b->plane_lds[1]);
+    pl.grid = b->grid_mode == DM_BATCH_GRID_PERSISTENT ? std::min<long long>((long long)pl.tiles.size(), res)
+                                                       : (long long)pl.tiles.size();
+  }
   return true;
 }
 
@@ -300,7 +363,7 @@ extern "C" int dm_batch_bind(DmBatch b, void *const *data) {
 
 extern "C" int dm_batch_set_schedule(DmBatch b, int tile, int radix_order) {
   if (b == nullptr) return 1;
-  b->tile_choice = tile == DM_BATCH_TILE_HALF ? DM_BATCH_TILE_HALF : DM_BATCH_TILE_FULL;
+  b->tile_choice = (tile == DM_BATCH_TILE_HALF || tile == DM_BATCH_TILE_PLANES) ? tile : DM_BATCH_TILE_FULL;
   b->radix_order = radix_order == DM_BATCH_RADIX_SMALL_FIRST ? DM_BATCH_RADIX_SMALL_FIRST
                                                               : DM_BATCH_RADIX_LARGE_FIRST;
   if (b->data.empty()) return 0;
@@ -318,6 +381,23 @@ extern "C" int dm_batch_run(DmBatch b, void *stream) {
   if (b == nullptr || b->passes.empty()) {
     raise_error(nullptr, "dm_batch_run: batch not bound (dm_batch_bind)");
     return 1;
+  }
+  for (int k = 0; k < 2; k++) {  // the last two axes of the plane transforms
+    const PassTables &pl = b->planes[k];
+    if (pl.tiles.empty()) continue;
+    dm::BatchLaunch L{};
+    L.tiles = (const dm::BatchTile *)pl.dev;
+    L.ntiles = (long long)pl.tiles.size();
+    L.sched = (const dm::BatchSched *)b->d_sched;
+    L.data = (void *const *)b->d_ptrs;
+    L.twiddle = b->d_tw;
+    L.dbl = b->dbl;
+    L.inverse = b->inverse;
+    L.primes = b->plane_primes[k];
+    L.plane = k == 0 ? 2 : 1;
+    L.grid = pl.grid;
+    L.lds = b->plane_lds[k];
+    if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run: plane pass launch")) return 1;
   }
   for (auto &p : b->passes) {
     dm::BatchLaunch L{};
@@ -386,7 +466,7 @@ extern "C" double dm_batch_autotune(DmBatch b, void *stream) {
   double best = 1e30;
   int bt = b->tile_choice, br = b->radix_order, bg = b->grid_mode;
   for (int grid : {DM_BATCH_GRID_TILES, DM_BATCH_GRID_PERSISTENT})
-  for (int tile : {DM_BATCH_TILE_FULL, DM_BATCH_TILE_HALF})
+  for (int tile : {DM_BATCH_TILE_FULL, DM_BATCH_TILE_HALF, DM_BATCH_TILE_PLANES})
     for (int order : {DM_BATCH_RADIX_LARGE_FIRST, DM_BATCH_RADIX_SMALL_FIRST}) {
       b->grid_mode = grid;
       if (dm_batch_set_schedule(b, tile, order) != 0) continue;
@@ -417,8 +497,9 @@ extern "C" int dm_batch_grid(DmBatch b) { return b == nullptr ? -1 : b->grid_mod
 extern "C" int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice, int *radix_order) {
   if (b == nullptr) return 1;
   for (size_t i = 0; i < b->passes.size(); i++) {
-    if (tiles) tiles[i] = (long long)b->passes[i].tiles.size();
-    if (grid) grid[i] = b->passes[i].grid;
+    const long long np = i == 0 ? (long long)(b->planes[0].tiles.size() + b->planes[1].tiles.size()) : 0;
+    if (tiles) tiles[i] = (long long)b->passes[i].tiles.size() + np;
+    if (grid) grid[i] = b->passes[i].grid + (i == 0 ? b->planes[0].grid + b->planes[1].grid : 0);
   }
   if (tile_choice) *tile_choice = b->tile_choice;
   if (radix_order) *radix_order = b->radix_order;

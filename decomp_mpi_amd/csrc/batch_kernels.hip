@@ -23,6 +23,15 @@ using gptr = __attribute__((address_space(1))) cx<R> *;  // global memory
 constexpr int BCAP_D = 4096;
 constexpr int BCAP_F = 4096;  // 8 points per thread in both precisions
 constexpr int BNTH = 512;
+// Plane tiles: a workgroup holds a whole plane of the last two axes in LDS.
+// Planes of up to the line tile's points run in the line kernel's shape (8
+// points per thread, 512 threads); complex-float planes of up to 8192
+// points at 16 points per thread on 512 threads (two workgroups per CU by
+// LDS).  (A 16384-point plane on 1024 threads, one workgroup per CU, took
+// 2.7 times as long per byte as a line pass: no memory work overlaps its
+// stages.)
+constexpr int PCAP_F = 8192;
+constexpr int PNTH = 512;
 // Twiddles (16 bytes per root) go to LDS after the tile when every schedule
 // of the launch has lines of at most BTW_LDS_N points; otherwise the launch
 // reads them from device memory.
@@ -137,8 +146,10 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 // only two); complex double spills below 104, so it stays at 4.
 // TWL: the schedules' twiddles are copied to LDS per tile (every line of
 // the launch at most BTW_LDS_N points).
-template <typename R, bool INV, int CAP, int NTH, bool PRIMES, bool TWL>
-__global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel(BatchLaunch p) {
+// PLANE: every tile is a plane tile (row 2): the rows' stages, then the
+// columns' stages on the same LDS tile, then one row-order store.
+template <typename R, bool INV, int CAP, int NTH, bool PRIMES, bool TWL, bool PLANE>
+__global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 4) void batch_pass_kernel(BatchLaunch p) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   constexpr int V = CAP / NTH;
@@ -151,7 +162,7 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     typedef typename tw_global<R>::type TwE;
     const TwE *twall;
     void *xdata;
-    if (p.nseg > 0) {
+    if (!PLANE && p.nseg > 0) {
       int sg = 0;
       while (sg + 1 < p.nseg && t >= p.seg[sg].end) sg++;
       const BatchSeg &g = p.seg[sg];
@@ -172,6 +183,7 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
       }
       d.xform = 0;
       d.sched = 0;
+      d.sched2 = 0;
       sched = g.sched;
       twall = reinterpret_cast<const TwE *>(g.twiddle);
       xdata = g.data;
@@ -193,6 +205,11 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     TwE *twl = reinterpret_cast<TwE *>(smem_raw + batch_tw_at(T, N, sizeof(cx<R>)));
     if constexpr (TWL)
       for (int i = threadIdx.x; i < N; i += NTH) twl[i] = tw[i];
+    if constexpr (PLANE && TWL) {  // the columns' roots after the rows'
+      const BatchSched &S2 = sched[d.sched2];
+      const TwE *tw2 = twall + S2.tw_off;
+      for (int i = threadIdx.x; i < S2.N; i += NTH) twl[N + i] = tw2[i];
+    }
     // Fused first / last stages: a power-of-two radix dividing the points per
     // thread (whole butterflies per thread); the last one needs two stages.
     const int r0 = S.radix[0], rl = S.radix[S.nstage - 1];
@@ -200,7 +217,7 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     // per-butterfly runs through the plain element walk: 1.27 vs 1.31 ms
     // per 256-transform batch)
     const bool fuse_in = (r0 == 2 || r0 == 4 || r0 == 8) && V % r0 == 0;
-    const bool fuse_out = S.nstage >= 2 && (rl == 2 || rl == 4 || rl == 8) && V % rl == 0;
+    const bool fuse_out = !PLANE && S.nstage >= 2 && (rl == 2 || rl == 4 || rl == 8) && V % rl == 0;
     if (fuse_in) {
       if (r0 == 8) load_first<R, INV, 8, CAP, NTH>(buf, S, d, data, lstride, ls);
       else if (r0 == 4) load_first<R, INV, 4, CAP, NTH>(buf, S, d, data, lstride, ls);
@@ -259,6 +276,30 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
     };
     if constexpr (TWL) run_stages((const lds_twg<R> *)twl);
     else run_stages(tw);
+    if constexpr (PLANE) {
+      // the columns: S2.T = N lines of S2.N = T points, point x of column l
+      // at x * ls + pad(l) (stage_lds ends on a barrier)
+      const BatchSched &S2 = sched[d.sched2];
+      auto run_cols = [&](auto tw2) {
+        typedef decltype(tw2) TW2;
+        for (int s = 0; s < S2.nstage; s++) {
+          switch (S2.radix[s]) {
+            case 2: stage_lds<R, INV, 2, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 3: stage_lds<R, INV, 3, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 4: stage_lds<R, INV, 4, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 5: if constexpr (PRIMES) stage_lds<R, INV, 5, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 7: if constexpr (PRIMES) stage_lds<R, INV, 7, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 8: stage_lds<R, INV, 8, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            case 16: stage_lds<R, INV, 16, CAP, NTH, CT, TW2, true>(buf, S2.st[s], S2.T, ls, tw2, S2.m_t); break;
+            default: break;
+          }
+        }
+      };
+      if constexpr (TWL) run_cols((const lds_twg<R> *)twl + N);
+      else run_cols(twall + S2.tw_off);
+    }
     if (!fuse_out) {
       // store: same element walk as the load, straight from LDS
 #pragma unroll
@@ -284,10 +325,10 @@ __global__ __launch_bounds__(NTH, sizeof(R) == 4 ? 8 : 4) void batch_pass_kernel
   }
 }
 
-template <typename R, bool INV, int CAP, bool PRIMES, bool TWL>
+template <typename R, bool INV, int CAP, bool PRIMES, bool TWL, bool PLANE = false>
 hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
-  constexpr int NTH = BNTH;
-  auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL>;
+  constexpr int NTH = CAP > BCAP_F ? PNTH : BNTH;
+  auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL, PLANE>;
   // once per kernel (a thread-safe static)
   static const hipError_t attr =
       hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -300,6 +341,14 @@ hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
 
 int batch_tile_capacity(int dbl) { return dbl ? BCAP_D : BCAP_F; }
 
+int batch_plane_capacity(int dbl) { return dbl ? BCAP_D : PCAP_F; }
+
+size_t batch_plane_lds_bytes(int dbl, int T, int N) {
+  if ((long long)T * N > batch_plane_capacity(dbl) || !batch_tw_in_lds(N) || !batch_tw_in_lds(T)) return 0;
+  const size_t b = batch_tw_at(T, N, dbl ? 16 : 8) + (size_t)(N + T) * 16;
+  return b <= 160 * 1024 ? b : 0;
+}
+
 bool batch_tw_in_lds(int N) { return N <= BTW_LDS_N; }
 
 size_t batch_lds_bytes(int dbl, int T, int N) {
@@ -310,10 +359,10 @@ size_t batch_lds_bytes(int dbl, int T, int N) {
 // (512 VGPRs per SIMD lane, 4 SIMDs per CU, at most 8 waves per SIMD).
 long long batch_resident_groups(int dbl, int primes, size_t lds) {
   const void *k;
-  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true, true>
-                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false, true>;
-  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true, true>
-                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false, true>;
+  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true, true, false>
+                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false, true, false>;
+  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true, true, false>
+                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false, true, false>;
   hipFuncAttributes fa;
   int vgprs = (hipFuncGetAttributes(&fa, k) == hipSuccess && fa.numRegs > 0) ? fa.numRegs : 128;
   int dev = 0, cus = 0;
@@ -326,16 +375,38 @@ long long batch_resident_groups(int dbl, int primes, size_t lds) {
   return (long long)cus * std::max(1, std::min(by_regs, by_lds));
 }
 
-template <typename R, int CAP, bool TWL>
+// Large-plane kernel: at most two workgroups per CU (128 VGPRs at 512
+// threads), fewer if LDS says so.
+long long batch_plane_resident_groups(size_t lds) {
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (cus <= 0) cus = 256;
+  return (long long)cus * std::max(1, std::min(2, (int)((160 * 1024) / std::max<size_t>(lds, 1))));
+}
+
+template <typename R, int CAP, bool TWL, bool PLANE = false>
 hipError_t launch_forms(const BatchLaunch &p, hipStream_t s) {
-  if (p.primes) return p.inverse ? launch_typed<R, true, CAP, true, TWL>(p, s) : launch_typed<R, false, CAP, true, TWL>(p, s);
-  return p.inverse ? launch_typed<R, true, CAP, false, TWL>(p, s) : launch_typed<R, false, CAP, false, TWL>(p, s);
+  if (p.primes)
+    return p.inverse ? launch_typed<R, true, CAP, true, TWL, PLANE>(p, s) : launch_typed<R, false, CAP, true, TWL, PLANE>(p, s);
+  return p.inverse ? launch_typed<R, true, CAP, false, TWL, PLANE>(p, s) : launch_typed<R, false, CAP, false, TWL, PLANE>(p, s);
 }
 
 int launch_batch_pass(const BatchLaunch &p, void *stream) {
   if (p.ntiles <= 0 || p.grid <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
+  // plane tiles: twiddles always in LDS (both axes at most BTW_LDS_N
+  // points); plane 2: planes within the line tile, plane 1: larger ones
+  // (complex float only)
+  if (p.plane == 2) {
+    e = p.dbl ? launch_forms<double, BCAP_D, true, true>(p, s) : launch_forms<float, BCAP_F, true, true>(p, s);
+    return (int)e;
+  }
+  if (p.plane == 1) {
+    if (p.dbl) return (int)hipErrorInvalidValue;
+    return (int)launch_forms<float, PCAP_F, true, true>(p, s);
+  }
   if (p.dbl) e = p.long_lines ? launch_forms<double, BCAP_D, false>(p, s) : launch_forms<double, BCAP_D, true>(p, s);
   else e = p.long_lines ? launch_forms<float, BCAP_F, false>(p, s) : launch_forms<float, BCAP_F, true>(p, s);
   return (int)e;

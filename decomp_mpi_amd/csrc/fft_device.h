@@ -261,11 +261,16 @@ __device__ __forceinline__ int lds_index(int line, int x, int ls) { return line 
 // TW: pointer to the twiddles, typed by where they live (an LDS-qualified
 // pointer reads with LDS instructions; a generic one would use flat loads);
 // CT: compensated twiddle products (complex float).
-template <typename R, bool INV, int RADIX, int CAP, int NTH, bool CT, typename TW>
-__device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T, int ls, TW tw) {
+// COL: the lines are the columns of the LDS tile (point x of line l at
+// x * ls + pad(l): a plane's second axis), consecutive butterflies on
+// consecutive lines; m_t: magic for T.
+template <typename R, bool INV, int RADIX, int CAP, int NTH, bool CT, typename TW, bool COL = false>
+__device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T, int ls, TW tw,
+                                          unsigned long long m_t = 0) {
   constexpr int K = (CAP / RADIX + NTH - 1) / NTH;
   const int nr = si.nr, ns = si.ns;
   const int nb = T * nr;
+  auto at = [&](int l, int x) { return COL ? x * ls + lds_pad(l) : l * ls + lds_pad(x); };
   cx<R> v[K][RADIX];
   int dst[K];
 #pragma unroll
@@ -273,16 +278,18 @@ __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T
     int b = threadIdx.x + k * NTH;
     dst[k] = -1;
     if (b < nb) {
-      int l = fdiv(b, si.m_nr);
-      int j = b - l * nr;
+      int l, j;
+      if constexpr (COL) {
+        j = fdiv(b, m_t);
+        l = b - j * T;
+      } else {
+        l = fdiv(b, si.m_nr);
+        j = b - l * nr;
+      }
       int jn = fdiv(j, si.m_ns);
       int jj = j - jn * ns;
-      int base = l * ls;
 #pragma unroll
-      for (int r = 0; r < RADIX; r++) {
-        int x = j + r * nr;
-        v[k][r] = buf[base + lds_pad(x)];
-      }
+      for (int r = 0; r < RADIX; r++) v[k][r] = buf[at(l, j + r * nr)];
       if (ns > 1) {
         // stage-ordered table: consecutive jj read consecutive entries
         TW tws = tw + (ns - 1) + jj;
@@ -300,12 +307,8 @@ __device__ __forceinline__ void stage_lds(cx<R> *buf, const StageInfo &si, int T
   for (int k = 0; k < K; k++) {
     if (dst[k] >= 0) {
       int l = dst[k] >> 16, idxd = dst[k] & 0xffff;
-      int base = l * ls;
 #pragma unroll
-      for (int r = 0; r < RADIX; r++) {
-        int x = idxd + r * ns;
-        buf[base + lds_pad(x)] = v[k][r];
-      }
+      for (int r = 0; r < RADIX; r++) buf[at(l, idxd + r * ns)] = v[k][r];
     }
   }
   __syncthreads();

@@ -139,6 +139,9 @@ int launch_scale(const ScaleLaunch &p, void *stream);
 // ---- K3: batched local FFT pass (many transforms, one axis of each) --------
 // A tile holds nl (<= T) lines of N points of one transform: point x of line
 // l at element base + l * (row ? N : 1) + x * ps of that transform's array.
+// row 2: a plane tile -- the last two axes of one transform at once, nl = T
+// rows of N contiguous points (schedule `sched`), then its N columns of T
+// points (schedule `sched2`, tile height N) in the same LDS tile.
 struct BatchTile {
   long long base;
   long long ps;
@@ -146,6 +149,7 @@ struct BatchTile {
   int sched;
   int nl;
   int row;
+  int sched2;
 };
 // Per (line length, tile height, radix order): stages and twiddles.
 struct BatchSched {
@@ -183,6 +187,7 @@ struct BatchLaunch {
   int dbl, inverse;
   int primes;                  // some schedule has a radix 5, 7, 11 or 13
   int long_lines;              // some schedule's lines are too long for twiddles in LDS
+  int plane;                   // plane tiles (row 2) only: 2 within the line tile's points, 1 larger
   long long grid;              // persistent workgroups (tile loop inside)
   size_t lds;
   int nseg;                    // > 0: tiles come from seg[] (tiles, sched, data, twiddle unused)
@@ -194,6 +199,11 @@ int launch_copy16(const void *src, void *dst, long long n16, void *stream);
 // schedules, their stage-ordered roots as complex doubles, LDS bytes.
 struct BatchHost {
   std::vector<std::vector<BatchTile>> tiles;
+  // plane tiles (run before tiles[0]): [0] planes within the line tile's
+  // points (the line kernel's shape), [1] larger ones
+  std::vector<BatchTile> planes[2];
+  size_t plane_lds[2] = {0, 0};
+  int plane_primes[2] = {0, 0};
   std::vector<BatchSched> sched;
   std::vector<double> tw;
   size_t lds = 0;
@@ -213,5 +223,13 @@ int batch_tile_height(int N, long long inner, int dbl, int tile_choice);
 size_t batch_lds_bytes(int dbl, int T, int N);
 bool batch_tw_in_lds(int N);
 long long batch_resident_groups(int dbl, int primes, size_t lds);
+// Plane tiles (BatchTile row 2): the most points a plane may hold, the LDS
+// bytes of a plane of T rows of N points (0 when it does not fit: too many
+// points, a side above the LDS-twiddle limit, or over 160 KiB), and the
+// resident workgroups of the large-plane kernel (planes above the line
+// tile's points) for a launch's LDS.
+int batch_plane_capacity(int dbl);
+size_t batch_plane_lds_bytes(int dbl, int T, int N);
+long long batch_plane_resident_groups(size_t lds);
 
 }  // namespace dm

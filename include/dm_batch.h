@@ -24,6 +24,11 @@ typedef struct dm_batch *DmBatch;
 /* Tile and radix schedule choices (dm_batch_set_schedule, dm_batch_autotune). */
 #define DM_BATCH_TILE_FULL 0    /* tiles of up to the kernel's capacity */
 #define DM_BATCH_TILE_HALF 1    /* tiles of up to half of it */
+/* The last two axes of a transform in one plane tile where the plane fits
+ * (at most 8192 complex-float or 4096 complex-double points, each side at
+ * most 256): one read and one write of those axes instead of two; full
+ * line tiles for the other axes and transforms. */
+#define DM_BATCH_TILE_PLANES 2
 #define DM_BATCH_RADIX_LARGE_FIRST 0
 #define DM_BATCH_RADIX_SMALL_FIRST 1
 /* Launch grids (dm_batch_set_grid, dm_batch_autotune). */
@@ -52,7 +57,8 @@ int dm_batch_graph_launch(DmBatch b, void *stream);
  *  it overwrites) and keeps the fastest; returns its milliseconds per run
  *  (< 0: error). */
 double dm_batch_autotune(DmBatch b, void *stream);
-/** Tiles per axis pass and launch grid of the current binding (inspection). */
+/** Tiles per axis pass and launch grid of the current binding (inspection;
+ *  ndim entries each: plane tiles are counted with the last axis's pass). */
 int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice, int *radix_order);
 /** Algorithmic HBM bytes of one run: sum over transforms of 2 * ndim * N * elem. */
 double dm_batch_bytes(DmBatch b);

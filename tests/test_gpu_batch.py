@@ -124,6 +124,45 @@ def test_batch_graph_replays_match_eager(C):
     assert max(errs) < 1e-6
 
 
+PLANE_CASES = [
+    # 3D: planes of the last two axes where they fit (complex float at most
+    # 8192 points, complex double 4096; the rest on line tiles), both plane
+    # sizes classes
+    SHAPES3,
+    # 2D: the plane is the whole transform; (512, 16) has a side above 256,
+    # (128, 128) is above either plane size
+    [(64, 48), (128, 128), (44, 52), (512, 16), (16, 208), (96, 80)],
+]
+
+
+@pytest.mark.parametrize("dbl", [1, 0])
+@pytest.mark.parametrize("direction", [-1, 1])
+@pytest.mark.parametrize("case", range(len(PLANE_CASES)))
+def test_batch_planes_match_numpy(C, case, dbl, direction):
+    """DM_BATCH_TILE_PLANES: the last two axes of each transform in one
+    plane tile (rows' stages, then the columns' in the same LDS tile), line
+    tiles for the rest; against numpy's FP64 DFT, and graph replays equal to
+    the eager run."""
+    b = Batch(C, PLANE_CASES[case], dbl, direction, seed=13)
+    assert C.dm_batch_set_schedule(b.plan, dm.DM_BATCH_TILE_PLANES, 0) == 0
+    tiles = (ctypes.c_longlong * b.ndim)()
+    tile = ctypes.c_int()
+    assert C.dm_batch_info(b.plan, tiles, None, ctypes.byref(tile), None) == 0
+    assert tile.value == dm.DM_BATCH_TILE_PLANES
+    print(f"planes: case {case} precision {dbl} direction {direction} tiles {list(tiles)}", flush=True)
+    b.upload()
+    assert C.dm_batch_run(b.plan, None) == 0
+    eager = b.download()
+    errs = [rel(g, w) for g, w in zip(eager, b.want())]
+    b.upload()
+    assert C.dm_batch_graph_launch(b.plan, None) == 0
+    C.dm_synchronize(b.comm)
+    replay = b.download()
+    b.free()
+    assert max(errs) < TOL[dbl], errs
+    assert all(np.array_equal(g, e) for g, e in zip(replay, eager))
+
+
 def test_batch_autotune_keeps_results(C):
     shapes = [(64, 96, 128), (32, 48, 64), (96, 32, 128)]
     b = Batch(C, shapes, 0, -1, seed=11)
