@@ -250,11 +250,12 @@ def pencil_bench(C, dm, steps=20, warmup=3, n=64, P=4):
             "tolerance": 1e-12, "ok": bool(err_p < 1e-12 and err_s < 1e-12)}
 
 
-def batched_bench(C, dm, steps, warmup):
+def batched_bench(C, dm, steps, warmup, schedule="auto"):
     """256 independent complex-float 3D FFTs of seeded uneven shapes (each
-    axis from {32, 48, 64, 96, 128}) in place, one batched launch per axis
-    replayed from a HIP graph; grid, tile and radix order from
-    dm_batch_autotune."""
+    axis from {32, 48, 64, 96, 128}) in place, the batched launches replayed
+    from a HIP graph; grid, tile and radix order from dm_batch_autotune, or
+    fixed (schedule "full" or "planes": one workgroup per tile, large
+    radices first) so that a profile sees one schedule throughout."""
     import ctypes
     from decomp_mpi_amd import workloads
     shapes = workloads.batched_shapes(256, seed=4)
@@ -271,7 +272,13 @@ def batched_bench(C, dm, steps, warmup):
     ptrs = (ctypes.c_void_p * len(shapes))(*[buf.ptr + o for o in offs])
     if not plan or C.dm_batch_bind(plan, ptrs) != 0:
         raise RuntimeError("batched plan failed")
-    tune_ms = C.dm_batch_autotune(plan, stream)
+    if schedule == "auto":
+        tune_ms = C.dm_batch_autotune(plan, stream)
+    else:
+        tile_id = {"full": dm.DM_BATCH_TILE_FULL, "planes": dm.DM_BATCH_TILE_PLANES}[schedule]
+        if C.dm_batch_set_schedule(plan, tile_id, 0) != 0:
+            raise RuntimeError("batched schedule failed")
+        tune_ms = -1.0
     tile, order = ctypes.c_int(), ctypes.c_int()
     C.dm_batch_info(plan, None, None, ctypes.byref(tile), ctypes.byref(order))
     grid_mode = C.dm_batch_grid(plan)
@@ -298,19 +305,23 @@ def batched_bench(C, dm, steps, warmup):
     dt = (time.perf_counter() - t0) / steps
     flops = sum(5 * np.prod(sh) * np.log2(np.prod(sh)) for sh in shapes)
     alg = C.dm_batch_bytes(plan)
+    moved = C.dm_batch_bytes_moved(plan)
     C.dm_batch_free(plan)
     buf.free()
     C.dm_world_free(comm)
     return {"workload": "256 complex-float 3D FFTs, axes from {32,48,64,96,128} (seed 4), in place, "
-                        "one HIP graph of 3 batched launches",
+                        "one HIP graph of the autotuned schedule's batched launches",
             "ms_per_batch": round(dt * 1e3, 4), "transforms_per_s": round(len(shapes) / dt, 1),
             "GFLOP_per_s": round(flops / dt / 1e9, 1), "GB_per_s": round(alg / dt / 1e9, 1),
             "frac_of_peak": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
+            "bytes_algorithmic": alg, "bytes_moved": moved,
+            "moved_GB_per_s": round(moved / dt / 1e9, 1),
             "max_rel_l2_vs_fp64": max_rel, "max_abs_err_over_max_vs_fp64": max_abs,
             "schedule": {"tile": ["full", "half", "planes (last two axes per plane tile where they fit)"][tile.value],
                          "radix_order": ["large_first", "small_first"][order.value],
                          "grid": ["one workgroup per tile", "persistent"][grid_mode],
-                         "autotune_best_ms": round(tune_ms, 4)}}
+                         "autotune_best_ms": round(tune_ms, 4) if tune_ms > 0 else None,
+                         "chosen_by": "dm_batch_autotune" if schedule == "auto" else "--batch-schedule " + schedule}}
 
 
 def serving_bench(C, dm, count=1000, streams=4, max_batch=8, budget=64 << 30):
@@ -539,6 +550,8 @@ def main():
     ap.add_argument("--no-reshape", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-schedule", choices=["auto", "full", "planes"], default="auto",
+                    help="batched section: autotuned schedule (default) or a fixed one")
     ap.add_argument("--factors", type=str, default=None,
                     help="B factors b0,b1,b2 for makeParallelPlan instead of the autotuner (profiling runs)")
     ap.add_argument("--only", choices=["batched", "reshape", "serving"], default=None,
@@ -555,7 +568,7 @@ def main():
         raise SystemExit("no HIP device")
     if args.only is not None:
         if args.only == "batched":
-            res = batched_bench(C, dm, max(args.steps, 5), max(args.warmup, 1))
+            res = batched_bench(C, dm, max(args.steps, 5), max(args.warmup, 1), args.batch_schedule)
             # a plain 1 GiB copy in the same run: the byte counters' calibration
             comm = C.dm_world_create(1)
             cb = dm.DistBuffer(comm, 2 << 30)
@@ -704,7 +717,7 @@ def main():
     batched = None
     if not args.no_batched:
         print("bench: batched 256 uneven FFTs", file=sys.stderr, flush=True)
-        batched = batched_bench(C, dm, max(args.steps, 5), max(args.warmup, 1))
+        batched = batched_bench(C, dm, max(args.steps, 5), max(args.warmup, 1), args.batch_schedule)
     serving = None
     if not args.no_serving:
         print("bench: serving loop", file=sys.stderr, flush=True)

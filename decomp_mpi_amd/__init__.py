@@ -145,6 +145,7 @@ _SIGS = {
     "dm_batch_run_noalloc": (I, [I, I, IP, I, I, ctypes.POINTER(P), P]),
     "dm_batch_set_grid": (I, [P, I]), "dm_batch_grid": (I, [P]),
     "dm_batch_info": (I, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), IP, IP]),
+    "dm_batch_bytes_moved": (D, [P]),
     "dm_batch_bytes": (D, [P]), "dm_batch_free": (None, [P]),
     # request loop (dm_serve.h)
     "dm_server_create": (P, [SZ, I, I]), "dm_server_submit": (ctypes.c_longlong, [P, I, IP, I, I, P, P]),

@@ -517,6 +517,24 @@ extern "C" double dm_batch_bytes(DmBatch b) {
   return total;
 }
 
+extern "C" double dm_batch_bytes_moved(DmBatch b) {
+  if (b == nullptr) return 0;
+  double total = 0;
+  for (int i = 0; i < b->count; i++) {
+    const int *n = &b->shapes[(size_t)i * b->ndim];
+    double pts = 1;
+    for (int d = 0; d < b->ndim; d++) pts *= n[d];
+    int passes = b->ndim;
+    // the plane rule of build_batch_host
+    if (b->tile_choice == DM_BATCH_TILE_PLANES && b->ndim >= 2) {
+      const int n1 = n[b->ndim - 2], n2 = n[b->ndim - 1], cap = dm::batch_tile_capacity(b->dbl);
+      if (dm::batch_plane_lds_bytes(b->dbl, n1, n2) > 0 && n1 <= cap && n2 <= cap) passes -= 1;
+    }
+    total += 2.0 * passes * pts * (b->dbl ? 16 : 8);
+  }
+  return total;
+}
+
 extern "C" void dm_batch_free(DmBatch b) {
   if (b == nullptr) return;
   release_bound(b);

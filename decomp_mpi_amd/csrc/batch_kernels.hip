@@ -147,9 +147,12 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 // TWL: the schedules' twiddles are copied to LDS per tile (every line of
 // the launch at most BTW_LDS_N points).
 // PLANE: every tile is a plane tile (row 2): the rows' stages, then the
-// columns' stages on the same LDS tile, then one row-order store.
+// columns' stages on the same LDS tile, then one row-order store.  Held to
+// 64 VGPRs the complex-float form spills (16 bytes per lane, 28% more bytes
+// written per launch), so its planes of up to the line tile's points get 6
+// waves per SIMD instead of 8; larger planes, and complex double, 4.
 template <typename R, bool INV, int CAP, int NTH, bool PRIMES, bool TWL, bool PLANE>
-__global__ __launch_bounds__(NTH, (sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 4) void batch_pass_kernel(BatchLaunch p) {
+__global__ __launch_bounds__(NTH, (PLANE && sizeof(R) == 4 && CAP / NTH == 8) ? 6 : ((sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 4)) void batch_pass_kernel(BatchLaunch p) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   constexpr int V = CAP / NTH;

@@ -62,6 +62,9 @@ double dm_batch_autotune(DmBatch b, void *stream);
 int dm_batch_info(DmBatch b, long long *tiles, long long *grid, int *tile_choice, int *radix_order);
 /** Algorithmic HBM bytes of one run: sum over transforms of 2 * ndim * N * elem. */
 double dm_batch_bytes(DmBatch b);
+/** Bytes the current schedule reads and writes: as dm_batch_bytes, with one
+ *  pass instead of two for the transforms whose last two axes run as planes. */
+double dm_batch_bytes_moved(DmBatch b);
 void dm_batch_free(DmBatch b);
 
 /** Device bytes of the fixed table area that dm_batch_run_noalloc uses. */
