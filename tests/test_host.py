@@ -323,3 +323,23 @@ def test_error_paths_record_mode(C):
     finally:
         C.dm_clear_error()
         C.dm_set_error_mode(0)
+
+
+def test_batch_plan_byte_counts(C):
+    """dm_batch_bytes counts one read and one write per axis of every
+    transform; with DM_BATCH_TILE_PLANES, dm_batch_bytes_moved counts one
+    pass fewer for each transform whose last two axes make a plane tile:
+    complex float up to 8192 points per plane, complex double 4096, every
+    side at most 256 (planning only, no device)."""
+    shapes = [(32, 48, 64), (64, 96, 128), (40, 128, 64), (8, 512, 8), (2, 16, 16)]
+    for dbl, plane_ok in ((0, [True, False, True, False, True]), (1, [True, False, False, False, True])):
+        es = 16 if dbl else 8
+        plan = C.dm_batch_plan(len(shapes), 3, dm.ints([v for s in shapes for v in s]), dbl, -1)
+        assert plan
+        alg = sum(2 * 3 * int(np.prod(s)) * es for s in shapes)
+        assert C.dm_batch_bytes(plan) == alg
+        assert C.dm_batch_bytes_moved(plan) == alg          # line tiles: three passes each
+        assert C.dm_batch_set_schedule(plan, dm.DM_BATCH_TILE_PLANES, 0) == 0
+        moved = sum(2 * (2 if ok else 3) * int(np.prod(s)) * es for s, ok in zip(shapes, plane_ok))
+        assert C.dm_batch_bytes_moved(plan) == moved
+        C.dm_batch_free(plan)
