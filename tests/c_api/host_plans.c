@@ -6,11 +6,16 @@
  * the other included), split FFT plans in both precisions from the slab
  * and the sqrt-factor planners, one refused plan in error mode 1, and every
  * object freed again.  Registered host memory stands in for the device
- * buffers, which planning never reads: nothing here needs a GPU.
+ * buffers, which planning never reads: nothing here needs a GPU.  Last,
+ * the 512^3 shapes on 32 ranks (the benchmark's; byte offsets above 2^31)
+ * over address space reserved without access rights, so that any read of
+ * the buffers while planning would fault.
  * Prints the number of plans made and "OK" when every check held. */
+#define _DEFAULT_SOURCE
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
 
 #include "block_decomp.h"
 #include "dm_runtime.h"
@@ -70,12 +75,19 @@ static void reshape_pair(MPI_Comm world, MPI_Datatype t, int es, Decomp a, Decom
   freeReshapePlan(r);
 }
 
-static void split_plans(MPI_Comm world, Decomp start, int P) {
+static void *reserve(size_t bytes) {
+  void *p = mmap(NULL, bytes, PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+  return p == MAP_FAILED ? NULL : p;
+}
+
+static void split_plans(MPI_Comm world, Decomp start, int P, int untouched) {
   int per = extent(start);
-  int length = 4 * per + 4096;
+  int length = untouched ? 2 * per : 4 * per + 4096;
   size_t es = sizeof(fftw_complex);
-  void *dat = calloc((size_t)P * per, es);
-  void *work = calloc((size_t)P * length, es);
+  void *dat = untouched ? reserve((size_t)P * per * es) : calloc((size_t)P * per, es);
+  void *work = untouched ? reserve((size_t)P * length * es) : calloc((size_t)P * length, es);
+  CHECK(dat != NULL && work != NULL, "buffers");
+  if (dat == NULL || work == NULL) return;
   CHECK(dm_register(world, dat, (size_t)per * es) == 0, "register dat");
   CHECK(dm_register(world, work, (size_t)length * es) == 0, "register work");
   for (int s = 0; s < 2; s++) {
@@ -97,8 +109,13 @@ static void split_plans(MPI_Comm world, Decomp start, int P) {
   }
   dm_unregister(dat);
   dm_unregister(work);
-  free(dat);
-  free(work);
+  if (untouched) {
+    munmap(dat, (size_t)P * per * es);
+    munmap(work, (size_t)P * length * es);
+  } else {
+    free(dat);
+    free(work);
+  }
 }
 
 int main(void) {
@@ -136,7 +153,7 @@ int main(void) {
           }
         }
       }
-      split_plans(world, start, P);
+      split_plans(world, start, P, 0);
 
       /* a reshape between arrays of different shapes is refused */
       int M[3] = {L[0] + 1, L[1], L[2]};
@@ -156,6 +173,22 @@ int main(void) {
       freeDecomp(start);
       dm_world_free(world);
     }
+  }
+  /* the benchmark's 512^3 on 32 ranks: the pencil start, the packed block
+   * start, a z-pencil, split plans in both precisions, 16-byte reshapes */
+  {
+    const int L[3] = {512, 512, 512}, P = 32;
+    MPI_Comm world = dm_world_create(P);
+    Decomp start = packed_block(world, L);
+    Decomp pen0 = pencil(world, L, 0), pen2 = pencil(world, L, 2);
+    reshape_pair(world, MPI_C_DOUBLE_COMPLEX, 16, start, pen2, "512^3 start<->pencil2");
+    reshape_pair(world, MPI_C_DOUBLE_COMPLEX, 16, pen0, pen2, "512^3 pencil0<->pencil2");
+    split_plans(world, pen0, P, 1);
+    split_plans(world, start, P, 1);
+    freeDecomp(start);
+    freeDecomp(pen0);
+    freeDecomp(pen2);
+    dm_world_free(world);
   }
   MPI_Type_free(&c48);
   printf("plans=%d\n%s\n", plans, failures ? "FAILED" : "OK");
