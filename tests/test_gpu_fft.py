@@ -2,8 +2,9 @@
 reference's two-stage transform (split_fft.c:249-682, dimscale.c), and the
 per-axis FFT stage against the oracle's reshape plus numpy's DFT.
 
-Tolerances: complex double within 1e-12 relative L2 (north-star bound vs an
-FP64 FFT); the FP32 form within 1e-6 relative L2 vs the FP64 oracle."""
+Tolerances: complex double within 1e-12 relative L2 (the stated bound
+against an FP64 FFT); the FP32 form within 1e-6 relative L2 vs the FP64
+oracle."""
 import numpy as np
 import pytest
 
