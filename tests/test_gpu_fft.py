@@ -264,26 +264,29 @@ def test_autotuned_plan_vs_oracle_and_dft(C, P, shape, grid, dbl):
     assert rt < tol, f"round trip {rt:.3e} (A={A})"
 
 
-def test_pencil_axis_ffts_64(C):
+@pytest.mark.parametrize("shape", [(64, 64, 64), (34, 19, 17)])
+def test_pencil_axis_ffts_64(C, shape):
     """x-pencil -> (FFT x) -> y-pencil -> (FFT y) -> z-pencil -> (FFT z):
     three fused passes, result equals numpy's 3D DFT placed by the z-pencil
-    decomposition; reshape placement checked through the oracle layout."""
-    n, P = 64, 4
+    decomposition; reshape placement checked through the oracle layout.
+    (34, 19, 17): uneven blocks and axes of primes above 13 (direct-sum
+    stages)."""
+    P = 4
     comm = C.dm_world_create(P)
 
     def pencil(ax):
         it = iter((2, 2))
-        return [("local", n) if d == ax else ("block", n, next(it)) for d in range(3)]
+        return [("local", shape[d]) if d == ax else ("block", shape[d], next(it)) for d in range(3)]
     px, ox = make_pair(C, comm, P, pencil(0))
     py, oy = make_pair(C, comm, P, pencil(1))
     pz, oz = make_pair(C, comm, P, pencil(2))
-    per = n * n * n // P
+    per = int(max(max(o.extents) for o in (ox, oy, oz)))
     b0 = dm.DistBuffer(comm, per * 16)
     b1 = dm.DistBuffer(comm, per * 16)
     rng = np.random.default_rng(3)
-    glob = rng.uniform(-1, 1, (n, n, n)) + 1j * rng.uniform(-1, 1, (n, n, n))
+    glob = rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
     host = np.zeros((P, per), np.complex128)
-    for c in np.ndindex(n, n, n):
+    for c in np.ndindex(*shape):
         r, off = ox.owner(c)
         host[r, off] = glob[c]
     b0.put_all(host)
@@ -296,10 +299,12 @@ def test_pencil_axis_ffts_64(C):
     got = b1.get_all(per * 16, np.complex128)
     ref = np.fft.fftn(glob)
     want = np.zeros_like(got)
-    for c in np.ndindex(n, n, n):
+    mask = np.zeros(got.shape, bool)
+    for c in np.ndindex(*shape):
         r, off = oz.owner(c)
         want[r, off] = ref[c]
-    assert rel_l2(got, want) < TOL64
+        mask[r, off] = True
+    assert rel_l2(got[mask], want[mask]) < TOL64
     for f in (fx, fy, fz):
         C.dm_free_axis_fft(f)
     b0.free()
