@@ -51,6 +51,7 @@ struct dm_batch {
   size_t lds = 0;
   int primes = 0;  // some schedule has a radix 5, 7, 11 or 13
   int long_lines = 0;  // some schedule's twiddles stay in device memory
+  int any = 0;         // some schedule has a direct-sum stage (a prime above 13)
   std::vector<PassTables> passes;
   // plane tiles (DM_BATCH_TILE_PLANES), run first: [0] within the line
   // tile's points, [1] larger
@@ -97,7 +98,7 @@ extern "C" DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int pre
   const int cap = dm::batch_tile_capacity(precision ? 1 : 0);
   for (long long i = 0; i < (long long)count * ndim; i++) {
     int n = shapes[i];
-    if (n < 1 || !dm::choose_radices(n, &rad) || rad.size() > 16 || n > cap) {
+    if (n < 1 || !dm::choose_radices(n, &rad, true) || rad.size() > 16 || n > cap) {
       raise_error(nullptr, "dm_batch_plan: axis length %d of transform %lld is not supported", n, i / ndim);
       return nullptr;
     }
@@ -132,7 +133,7 @@ int batch_tile_height(int N, long long inner, int dbl, int tile_choice) {
 void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vector<double> *tw) {
   BatchSched S{};
   std::vector<int> rad;
-  choose_radices(N, &rad);
+  choose_radices(N, &rad, true);
   if (radix_order == DM_BATCH_RADIX_SMALL_FIRST) std::reverse(rad.begin(), rad.end());
   S.N = N;
   S.nstage = (int)rad.size();
@@ -160,7 +161,24 @@ void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vecto
       }
     ns *= R;
   }
+  // after the N stage entries: w_p^m = w^(m N / p), m < p, for each stage
+  // of a prime radix without a butterfly kernel (direct sums), in stage order
+  for (int s = 0; s < S.nstage; s++) {
+    const int R = S.radix[s];
+    if (has_radix_kernel(R)) continue;
+    for (int m = 0; m < R; m++) {
+      const size_t k = (size_t)m * (N / R);
+      tw->push_back(root[2 * k]);
+      tw->push_back(root[2 * k + 1]);
+    }
+  }
   *out = S;
+}
+
+bool batch_sched_any(const BatchSched &S) {
+  for (int s = 0; s < S.nstage; s++)
+    if (!has_radix_kernel(S.radix[s])) return true;
+  return false;
 }
 
 // Host image of a batch: tiles per axis pass (last axis first), schedules
@@ -197,6 +215,7 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
   auto line_schedule = [&](int N, int T) -> int {
     int id = schedule(N, T);
     if (has_primes(h.sched[id])) h.primes = 1;
+    if (batch_sched_any(h.sched[id])) h.any = 1;
     if (!batch_tw_in_lds(N)) h.long_lines = 1;
     h.lds = std::max(h.lds, batch_lds_bytes(dbl, T, N));
     return id;
@@ -207,7 +226,12 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
     int dlast = ndim;  // axes [0, dlast) get line tiles
     if (tile_choice == DM_BATCH_TILE_PLANES && ndim >= 2) {
       const int n1 = n[ndim - 2], n2 = n[ndim - 1];
-      const size_t plds = batch_plane_lds_bytes(dbl, n1, n2);
+      size_t plds = batch_plane_lds_bytes(dbl, n1, n2);
+      std::vector<int> r1, r2;  // planes take butterfly-kernel radices only
+      choose_radices(n1, &r1, true);
+      choose_radices(n2, &r2, true);
+      for (int r : r1) plds = has_radix_kernel(r) ? plds : 0;
+      for (int r : r2) plds = has_radix_kernel(r) ? plds : 0;
       if (plds > 0 && n1 <= cap && n2 <= cap) {
         long long outer = 1;
         for (int e = 0; e < ndim - 2; e++) outer *= n[e];
@@ -282,6 +306,7 @@ static bool bind_tables(DmBatch b) {
   b->lds = h.lds;
   b->primes = h.primes;
   b->long_lines = h.long_lines;
+  b->any = h.any;
   b->passes.assign(b->ndim, PassTables());
   for (int d = 0; d < b->ndim; d++) b->passes[d].tiles = std::move(h.tiles[d]);
   for (int k = 0; k < 2; k++) {
@@ -410,6 +435,7 @@ extern "C" int dm_batch_run(DmBatch b, void *stream) {
     L.inverse = b->inverse;
     L.primes = b->primes;
     L.long_lines = b->long_lines;
+    L.any = b->any;
     L.grid = p.grid;
     L.lds = b->lds;
     if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run: batched pass launch")) return 1;
@@ -528,7 +554,13 @@ extern "C" double dm_batch_bytes_moved(DmBatch b) {
     // the plane rule of build_batch_host
     if (b->tile_choice == DM_BATCH_TILE_PLANES && b->ndim >= 2) {
       const int n1 = n[b->ndim - 2], n2 = n[b->ndim - 1], cap = dm::batch_tile_capacity(b->dbl);
-      if (dm::batch_plane_lds_bytes(b->dbl, n1, n2) > 0 && n1 <= cap && n2 <= cap) passes -= 1;
+      std::vector<int> r1, r2;
+      dm::choose_radices(n1, &r1, true);
+      dm::choose_radices(n2, &r2, true);
+      bool kern = true;
+      for (int r : r1) kern = kern && dm::has_radix_kernel(r);
+      for (int r : r2) kern = kern && dm::has_radix_kernel(r);
+      if (kern && dm::batch_plane_lds_bytes(b->dbl, n1, n2) > 0 && n1 <= cap && n2 <= cap) passes -= 1;
     }
     total += 2.0 * passes * pts * (b->dbl ? 16 : 8);
   }

@@ -151,7 +151,9 @@ __device__ __forceinline__ void last_store(const cx<R> *buf, const BatchSched &S
 // 64 VGPRs the complex-float form spills (16 bytes per lane, 28% more bytes
 // written per launch), so its planes of up to the line tile's points get 6
 // waves per SIMD instead of 8; larger planes, and complex double, 4.
-template <typename R, bool INV, int CAP, int NTH, bool PRIMES, bool TWL, bool PLANE>
+// ANY: some schedule has a direct-sum stage (a prime above 13; line tiles
+// only), a kernel of its own so that the others keep their registers.
+template <typename R, bool INV, int CAP, int NTH, bool PRIMES, bool TWL, bool PLANE, bool ANY>
 __global__ __launch_bounds__(NTH, (PLANE && sizeof(R) == 4 && CAP / NTH == 8) ? 6 : ((sizeof(R) == 4 && CAP / NTH == 8) ? 8 : 4)) void batch_pass_kernel(BatchLaunch p) {
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
@@ -257,6 +259,10 @@ __global__ __launch_bounds__(NTH, (PLANE && sizeof(R) == 4 && CAP / NTH == 8) ? 
     __syncthreads();
     const int s0 = fuse_in ? 1 : 0, s1 = fuse_out ? S.nstage - 1 : S.nstage;
     auto run_stages = [&](auto tw) {
+      // roots of the direct-sum stages: after the schedule's N stage
+      // entries, p per such stage in stage order (a fused first stage is a
+      // power of two, so none is skipped)
+      int roff = N;
       for (int s = s0; s < s1; s++) {
         switch (S.radix[s]) {
           case 2: stage_lds<R, INV, 2, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
@@ -268,7 +274,12 @@ __global__ __launch_bounds__(NTH, (PLANE && sizeof(R) == 4 && CAP / NTH == 8) ? 
           case 11: if constexpr (PRIMES) stage_lds<R, INV, 11, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
           case 13: if constexpr (PRIMES) stage_lds<R, INV, 13, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
           case 16: stage_lds<R, INV, 16, CAP, NTH, CT>(buf, S.st[s], T, ls, tw); break;
-          default: break;
+          default:
+            if constexpr (ANY) {
+              stage_lds_any<R, INV, CAP, NTH, CT>(buf, S.st[s], S.radix[s], T, ls, tw, twall + S.tw_off + roff);
+              roff += S.radix[s];
+            }
+            break;
         }
       }
       if (fuse_out) {
@@ -328,10 +339,10 @@ __global__ __launch_bounds__(NTH, (PLANE && sizeof(R) == 4 && CAP / NTH == 8) ? 
   }
 }
 
-template <typename R, bool INV, int CAP, bool PRIMES, bool TWL, bool PLANE = false>
+template <typename R, bool INV, int CAP, bool PRIMES, bool TWL, bool PLANE = false, bool ANY = false>
 hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
   constexpr int NTH = CAP > BCAP_F ? PNTH : BNTH;
-  auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL, PLANE>;
+  auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL, PLANE, ANY>;
   // once per kernel (a thread-safe static)
   static const hipError_t attr =
       hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -362,10 +373,10 @@ size_t batch_lds_bytes(int dbl, int T, int N) {
 // (512 VGPRs per SIMD lane, 4 SIMDs per CU, at most 8 waves per SIMD).
 long long batch_resident_groups(int dbl, int primes, size_t lds) {
   const void *k;
-  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true, true, false>
-                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false, true, false>;
-  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true, true, false>
-                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false, true, false>;
+  if (dbl) k = primes ? (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, true, true, false, false>
+                      : (const void *)batch_pass_kernel<double, false, BCAP_D, BNTH, false, true, false, false>;
+  else k = primes ? (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, true, true, false, false>
+                  : (const void *)batch_pass_kernel<float, false, BCAP_F, BNTH, false, true, false, false>;
   hipFuncAttributes fa;
   int vgprs = (hipFuncGetAttributes(&fa, k) == hipSuccess && fa.numRegs > 0) ? fa.numRegs : 128;
   int dev = 0, cus = 0;
@@ -390,6 +401,10 @@ long long batch_plane_resident_groups(size_t lds) {
 
 template <typename R, int CAP, bool TWL, bool PLANE = false>
 hipError_t launch_forms(const BatchLaunch &p, hipStream_t s) {
+  if constexpr (!PLANE)
+    if (p.any)
+      return p.inverse ? launch_typed<R, true, CAP, true, TWL, false, true>(p, s)
+                       : launch_typed<R, false, CAP, true, TWL, false, true>(p, s);
   if (p.primes)
     return p.inverse ? launch_typed<R, true, CAP, true, TWL, PLANE>(p, s) : launch_typed<R, false, CAP, true, TWL, PLANE>(p, s);
   return p.inverse ? launch_typed<R, true, CAP, false, TWL, PLANE>(p, s) : launch_typed<R, false, CAP, false, TWL, PLANE>(p, s);

@@ -82,7 +82,7 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
   const int cap = dm::batch_tile_capacity(dbl);
   for (long long i = 0; i < (long long)count * ndim; i++) {
     const int n = shapes[i];
-    if (n < 1 || n > cap || !dm::choose_radices(n, &rad) || rad.size() > 16) {
+    if (n < 1 || n > cap || !dm::choose_radices(n, &rad, true) || rad.size() > 16) {
       raise_error(nullptr, "dm_batch_run_noalloc: axis length %d of transform %lld is not supported", n, i / ndim);
       return 1;
     }
@@ -145,6 +145,7 @@ extern "C" int dm_batch_run_noalloc(int count, int ndim, const int *shapes, int 
     L.inverse = direction == 1 ? 1 : 0;
     L.primes = h.primes;
     L.long_lines = h.long_lines;
+    L.any = h.any;
     L.grid = L.ntiles;  // one workgroup per tile, as the bound form's default
     L.lds = h.lds;
     if (!dm::launch_ok(nullptr, dm::launch_batch_pass(L, stream), "dm_batch_run_noalloc: batched pass launch"))

@@ -188,6 +188,7 @@ struct BatchLaunch {
   int primes;                  // some schedule has a radix 5, 7, 11 or 13
   int long_lines;              // some schedule's lines are too long for twiddles in LDS
   int plane;                   // plane tiles (row 2) only: 2 within the line tile's points, 1 larger
+  int any;                     // some schedule has a direct-sum stage (line tiles only)
   long long grid;              // persistent workgroups (tile loop inside)
   size_t lds;
   int nseg;                    // > 0: tiles come from seg[] (tiles, sched, data, twiddle unused)
@@ -209,12 +210,17 @@ struct BatchHost {
   size_t lds = 0;
   int primes = 0;
   int long_lines = 0;
+  int any = 0;  // some line schedule has a direct-sum stage (a prime above 13)
 };
 bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_choice, int radix_order,
                       BatchHost *out);
-// One schedule: lines of N points (radices 2..16, 3, 5, 7, 11, 13) in tiles
-// of T lines, tw_off 0; tw: its N stage-ordered roots as (re, im) doubles.
+// One schedule: lines of N points (radices 2..16, 3, 5, 7, 11, 13 by
+// butterfly kernels, larger primes by direct sums) in tiles of T lines,
+// tw_off 0; tw: its N stage-ordered roots as (re, im) doubles, then the p
+// roots of each direct-sum stage in stage order.
 void make_batch_sched(int N, int T, int radix_order, BatchSched *out, std::vector<double> *tw);
+// Whether a schedule has a direct-sum stage.
+bool batch_sched_any(const BatchSched &S);
 int batch_tile_capacity(int dbl);
 int batch_tile_height(int N, long long inner, int dbl, int tile_choice);
 // LDS of a tile of T lines of N points, with the schedule's twiddles when

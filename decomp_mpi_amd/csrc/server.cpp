@@ -56,6 +56,7 @@ struct Sched {
   const void *tw = nullptr;
   size_t lds = 0;
   int primes = 0;
+  int any = 0;              // a direct-sum stage (a prime above 13)
   int long_lines = 0;       // lines too long for twiddles in LDS
   int refs = 0;             // requests admitted with this schedule and not yet finished
   long long last_use = 0;
@@ -70,6 +71,7 @@ struct ReqPlan {
   long long ntiles[3] = {0, 0, 0};
   size_t lds = 0;
   int primes = 0;
+  int any = 0;
   int long_lines = 0;
 };
 
@@ -198,11 +200,12 @@ static Sched *sched_for(DmServer s, int N, int T, int dbl) {
   dm::BatchSched one{};
   std::vector<double> tw;
   dm::make_batch_sched(N, T, DM_BATCH_RADIX_LARGE_FIRST, &one, &tw);
-  const size_t tw_bytes = (size_t)N * 16;
+  const size_t ntw = tw.size() / 2;  // N stage roots, then any direct-sum stages' roots
+  const size_t tw_bytes = ntw * 16;
   const size_t o_tw = align256(sizeof one);
   std::vector<char> img(o_tw + tw_bytes, 0);
   memcpy(img.data(), &one, sizeof one);
-  dm::twiddle_image(tw.data(), (size_t)N, dbl, img.data() + o_tw);
+  dm::twiddle_image(tw.data(), ntw, dbl, img.data() + o_tw);
   Sched *p = new Sched;
   p->bytes = img.size();
   p->dev = dm::device_alloc(p->bytes);
@@ -218,6 +221,7 @@ static Sched *sched_for(DmServer s, int N, int T, int dbl) {
   p->long_lines = dm::batch_tw_in_lds(N) ? 0 : 1;
   for (int k = 0; k < one.nstage; k++)
     if (one.radix[k] == 5 || one.radix[k] == 7 || one.radix[k] == 11 || one.radix[k] == 13) p->primes = 1;
+  p->any = dm::batch_sched_any(one) ? 1 : 0;
   p->last_use = ++s->sched_clock;
   p->refs = 1;  // held by the caller, so the trim below keeps it
   s->scheds[key] = p;
@@ -273,6 +277,7 @@ static bool plan_request(DmServer s, Req &r) {
     }
     pl.lds = std::max(pl.lds, sc->lds);
     pl.primes = pl.primes || sc->primes;
+    pl.any = pl.any || sc->any;
     pl.long_lines = pl.long_lines || sc->long_lines;
   }
   r.planned = true;
@@ -321,7 +326,7 @@ extern "C" long long dm_server_submit(DmServer s, int ndim, const int *shape, in
   std::vector<int> rad;
   for (int d = 0; d < ndim; d++) {
     r.shape[d] = shape[d];
-    if (shape[d] < 1 || shape[d] > dm::batch_tile_capacity(r.dbl) || !dm::choose_radices(shape[d], &rad) ||
+    if (shape[d] < 1 || shape[d] > dm::batch_tile_capacity(r.dbl) || !dm::choose_radices(shape[d], &rad, true) ||
         rad.size() > 16) {
       reject("dm_server_submit: axis length %d is not supported", shape[d]);
       return -1;
@@ -411,10 +416,11 @@ static bool launch(DmServer s, Slot &sl) {
   }
   ok = ok && hipEventRecord(sl.ev[1], st) == hipSuccess;
   size_t lds = 0;
-  int primes = 0, long_lines = 0;
+  int primes = 0, long_lines = 0, any = 0;
   for (long long id : ids) {
     lds = std::max(lds, s->reqs[id].plan.lds);
     primes = primes || s->reqs[id].plan.primes;
+    any = any || s->reqs[id].plan.any;
     long_lines = long_lines || s->reqs[id].plan.long_lines;
   }
   for (int d = 0; d < nd && ok; d++) {
@@ -423,6 +429,7 @@ static bool launch(DmServer s, Slot &sl) {
     L.inverse = inv;
     L.primes = primes;
     L.long_lines = long_lines;
+    L.any = any;
     L.lds = lds;
     L.nseg = (int)ids.size();
     long long end = 0;

@@ -36,8 +36,9 @@ typedef struct dm_batch *DmBatch;
 #define DM_BATCH_GRID_PERSISTENT 1  /* as many workgroups as fit the device at once, each looping over tiles */
 
 /** count transforms of ndim (1..3) axes; shapes[i * ndim + d] is axis d of
- *  transform i.  Axis lengths must factor into 2, 3, 5, 7, 11, 13 with at
- *  most 16 stages.  Returns NULL (after an error message) otherwise. */
+ *  transform i.  Axis lengths up to 4096 points, any factors (primes above
+ *  13 run as direct-sum stages, O(p) work per point), at most 16 stages.
+ *  Returns NULL (after an error message) otherwise. */
 DmBatch dm_batch_plan(int count, int ndim, const int *shapes, int precision, int direction);
 /** Device pointers of the transforms' data, data[i] for transform i.  Builds
  *  and uploads the tile tables; must precede run, graph and autotune. */

@@ -76,6 +76,8 @@ def rel(a, b):
 
 SHAPES3 = [(32, 48, 64), (64, 96, 128), (128, 32, 48), (96, 96, 32), (48, 64, 128), (32, 32, 32),
            (80, 112, 48), (176, 32, 16), (208, 16, 32), (16, 240, 32)]
+# axes with primes above 13 (direct-sum stages), beside ordinary ones
+SHAPES_PRIME = [(17, 34, 16), (19, 32, 23), (64, 17, 48), (32, 32, 32), (127, 2, 8), (16, 16, 289)]
 
 
 @pytest.mark.parametrize("dbl", [1, 0])
@@ -86,6 +88,32 @@ def test_batch_3d_matches_numpy(C, dbl, direction):
     assert C.dm_batch_run(b.plan, None) == 0
     got, want = b.download(), b.want()
     errs = [rel(g, w) for g, w in zip(got, want)]
+    b.free()
+    assert max(errs) < TOL[dbl], errs
+
+
+@pytest.mark.parametrize("dbl", [1, 0])
+@pytest.mark.parametrize("direction", [-1, 1])
+@pytest.mark.parametrize("tile", [0, 2])
+def test_batch_prime_axes(C, dbl, direction, tile):
+    """Batches whose axes include primes above 13 (the direct-sum kernel
+    for those launches), in the full and plane schedules (planes are not
+    formed for such axes), against numpy; and the no-alloc entry point on
+    the same inputs bit for bit equal to the full schedule."""
+    b = Batch(C, SHAPES_PRIME, dbl, direction, seed=17)
+    assert C.dm_batch_set_schedule(b.plan, tile, 0) == 0
+    b.upload()
+    assert C.dm_batch_run(b.plan, None) == 0
+    got = b.download()
+    errs = [rel(g, w) for g, w in zip(got, b.want())]
+    if tile == 0:
+        b.upload()
+        flat = dm.ints([v for s in b.shapes for v in s])
+        ptrs = (ctypes.c_void_p * len(b.shapes))(*[b.buf.ptr + o for o in b.offs])
+        assert C.dm_batch_run_noalloc(len(b.shapes), b.ndim, flat, dbl, direction, ptrs, None) == 0
+        C.dm_synchronize(b.comm)
+        again = b.download()
+        assert all(np.array_equal(g, a) for g, a in zip(got, again))
     b.free()
     assert max(errs) < TOL[dbl], errs
 

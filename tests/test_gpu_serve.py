@@ -213,14 +213,14 @@ def test_server_plans_stay_on_the_device(C):
 
 
 def test_server_refuses_without_abort(C):
-    """A request the server cannot take (an axis with a prime factor above
-    13) returns -1 with the reason in dm_last_error, in the default error
-    mode, and the server keeps serving."""
+    """A request the server cannot take (an axis longer than one tile of
+    the batched kernel) returns -1 with the reason in dm_last_error, in the
+    default error mode, and the server keeps serving."""
     s = C.dm_server_create(1 << 30, 1, 2)
-    x = np.zeros((34, 32, 32), np.complex64)
+    x = np.zeros((8192, 2, 2), np.complex64)
     pin = Pinned(C, x)
     C.dm_clear_error()
-    assert C.dm_server_submit(s, 3, dm.ints([34, 32, 32]), 0, -1, pin.ptr, pin.ptr) == -1
+    assert C.dm_server_submit(s, 3, dm.ints([8192, 2, 2]), 0, -1, pin.ptr, pin.ptr) == -1
     assert b"not supported" in C.dm_last_error()
     C.dm_clear_error()
     y = workloads.uniform_complex(np.random.default_rng(3), (32, 32, 32), 1)
@@ -230,3 +230,23 @@ def test_server_refuses_without_abort(C):
     C.dm_server_destroy(s)
     pin.free()
     release([req])
+
+
+def test_server_prime_axes(C):
+    """Axes with primes above 13 (direct-sum stages) in both precisions and
+    directions, mixed with ordinary ones in the same batches."""
+    s = C.dm_server_create(1 << 30, 2, 4)
+    rng = np.random.default_rng(31)
+    reqs = []
+    for shape, prec, direction in (((34, 32, 17), 0, -1), ((19, 16, 32), 1, 1), ((32, 32, 32), 0, 1),
+                                   ((23, 46, 16), 1, -1), ((48, 17, 64), 0, -1)):
+        x = workloads.uniform_complex(rng, shape, prec)
+        reqs.append((shape, prec, direction, x, Pinned(C, x), Pinned(C, np.zeros_like(x))))
+    ids = [submit(C, s, r) for r in reqs]
+    for rid in ids:
+        assert C.dm_server_wait(s, rid) == 0
+    for r in reqs:
+        err = np.linalg.norm(r[5].array() - want(r)) / np.linalg.norm(want(r))
+        assert err < TOL[r[1]], (r[0], r[1], err)
+    C.dm_server_destroy(s)
+    release(reqs)
