@@ -264,14 +264,16 @@ def test_autotuned_plan_vs_oracle_and_dft(C, P, shape, grid, dbl):
     assert rt < tol, f"round trip {rt:.3e} (A={A})"
 
 
+@pytest.mark.parametrize("dbl", [1, 0])
 @pytest.mark.parametrize("shape", [(64, 64, 64), (34, 19, 17)])
-def test_pencil_axis_ffts_64(C, shape):
+def test_pencil_axis_ffts_64(C, shape, dbl):
     """x-pencil -> (FFT x) -> y-pencil -> (FFT y) -> z-pencil -> (FFT z):
     three fused passes, result equals numpy's 3D DFT placed by the z-pencil
     decomposition; reshape placement checked through the oracle layout.
     (34, 19, 17): uneven blocks and axes of primes above 13 (direct-sum
-    stages)."""
+    stages).  Complex float within the FP32 tolerance of the FP64 DFT."""
     P = 4
+    es, ct = (16, np.complex128) if dbl else (8, np.complex64)
     comm = C.dm_world_create(P)
 
     def pencil(ax):
@@ -281,22 +283,25 @@ def test_pencil_axis_ffts_64(C, shape):
     py, oy = make_pair(C, comm, P, pencil(1))
     pz, oz = make_pair(C, comm, P, pencil(2))
     per = int(max(max(o.extents) for o in (ox, oy, oz)))
-    b0 = dm.DistBuffer(comm, per * 16)
-    b1 = dm.DistBuffer(comm, per * 16)
+    b0 = dm.DistBuffer(comm, per * es)
+    b1 = dm.DistBuffer(comm, per * es)
     rng = np.random.default_rng(3)
     glob = rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
-    host = np.zeros((P, per), np.complex128)
+    if not dbl:
+        glob = glob.astype(np.complex64).astype(np.complex128)
+    host = np.zeros((P, per), ct)
     for c in np.ndindex(*shape):
         r, off = ox.owner(c)
         host[r, off] = glob[c]
     b0.put_all(host)
-    fx = C.dm_plan_axis_fft(px, py, 0, 1, dm.FFTW_FORWARD)
-    fy = C.dm_plan_axis_fft(py, pz, 1, 1, dm.FFTW_FORWARD)
-    fz = C.dm_plan_axis_fft(pz, pz, 2, 1, dm.FFTW_FORWARD)
+    fx = C.dm_plan_axis_fft(px, py, 0, dbl, dm.FFTW_FORWARD)
+    fy = C.dm_plan_axis_fft(py, pz, 1, dbl, dm.FFTW_FORWARD)
+    fz = C.dm_plan_axis_fft(pz, pz, 2, dbl, dm.FFTW_FORWARD)
+    assert fx and fy and fz
     assert C.dm_run_axis_fft(fx, b0.ptr, b1.ptr) == 0
     assert C.dm_run_axis_fft(fy, b1.ptr, b0.ptr) == 0
     assert C.dm_run_axis_fft(fz, b0.ptr, b1.ptr) == 0
-    got = b1.get_all(per * 16, np.complex128)
+    got = b1.get_all(per * es, ct).astype(np.complex128)
     ref = np.fft.fftn(glob)
     want = np.zeros_like(got)
     mask = np.zeros(got.shape, bool)
@@ -304,7 +309,7 @@ def test_pencil_axis_ffts_64(C, shape):
         r, off = oz.owner(c)
         want[r, off] = ref[c]
         mask[r, off] = True
-    assert rel_l2(got[mask], want[mask]) < TOL64
+    assert rel_l2(got[mask], want[mask]) < (TOL64 if dbl else TOL32)
     for f in (fx, fy, fz):
         C.dm_free_axis_fft(f)
     b0.free()
