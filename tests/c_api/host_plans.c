@@ -18,6 +18,7 @@
 #include <sys/mman.h>
 
 #include "block_decomp.h"
+#include "dm_batch.h"
 #include "dm_runtime.h"
 #include "local_decomp.h"
 #include "processor_grid.h"
@@ -189,6 +190,26 @@ int main(void) {
     freeDecomp(pen0);
     freeDecomp(pen2);
     dm_world_free(world);
+  }
+  /* batched plans (host side only: validation, schedules, byte counts) */
+  {
+    static const int shapes[] = {17, 34, 16, 19, 32, 23, 64, 96, 128, 4096, 1, 1, 2, 16, 289};
+    for (int dbl = 0; dbl < 2; dbl++) {
+      DmBatch b = dm_batch_plan(5, 3, shapes, dbl, -1);
+      CHECK(b != NULL, "batch plan precision %d", dbl);
+      if (b == NULL) continue;
+      const double alg = dm_batch_bytes(b);
+      CHECK(dm_batch_bytes_moved(b) == alg, "line tiles move the algorithmic bytes");
+      CHECK(dm_batch_set_schedule(b, DM_BATCH_TILE_PLANES, DM_BATCH_RADIX_SMALL_FIRST) == 0, "plane schedule");
+      CHECK(dm_batch_bytes_moved(b) < alg, "planes move fewer bytes");
+      dm_batch_free(b);
+      plans++;
+    }
+    static const int too_long[] = {8192, 2, 2};
+    dm_set_error_mode(1);
+    CHECK(dm_batch_plan(1, 3, too_long, 0, -1) == NULL, "an axis longer than the batched tile is refused");
+    dm_clear_error();
+    dm_set_error_mode(0);
   }
   MPI_Type_free(&c48);
   printf("plans=%d\n%s\n", plans, failures ? "FAILED" : "OK");

@@ -76,8 +76,10 @@ def rel(a, b):
 
 SHAPES3 = [(32, 48, 64), (64, 96, 128), (128, 32, 48), (96, 96, 32), (48, 64, 128), (32, 32, 32),
            (80, 112, 48), (176, 32, 16), (208, 16, 32), (16, 240, 32)]
-# axes with primes above 13 (direct-sum stages), beside ordinary ones
-SHAPES_PRIME = [(17, 34, 16), (19, 32, 23), (64, 17, 48), (32, 32, 32), (127, 2, 8), (16, 16, 289)]
+# axes with primes above 13 (direct-sum stages), beside ordinary ones and
+# axes of length 1
+SHAPES_PRIME = [(17, 34, 16), (19, 32, 23), (64, 17, 48), (32, 32, 32), (127, 2, 8), (16, 16, 289),
+                (1, 17, 32), (8, 1, 1), (4, 1, 64)]
 
 
 @pytest.mark.parametrize("dbl", [1, 0])
