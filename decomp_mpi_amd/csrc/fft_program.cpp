@@ -182,10 +182,12 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   L.out_row = out_row ? 1 : 0;
   if (u >= 0) {
     // column walk at a line stride of 64 KiB or more: wide rows pay for the
-    // lower occupancy of the wide tile
+    // lower occupancy of the wide tile -- for lines of 256 points or more
+    // (the phase order's 16- and 32-point sub-axis passes ran slower in it:
+    // FFT_2 6.15 -> 5.54 ms per 512^3 step without)
     const size_t es = dbl ? 16 : 8;
-    bool wide = (!in_row && (size_t)I.h.m[t] * es >= (64u << 10)) ||
-                (!out_row && (size_t)U.h.m[t] * es >= (64u << 10));
+    bool wide = L.N >= 256 && ((!in_row && (size_t)I.h.m[t] * es >= (64u << 10)) ||
+                               (!out_row && (size_t)U.h.m[t] * es >= (64u << 10)));
     static const bool nowide = getenv("DM_NO_WIDE") != nullptr;
     int tcap = fft_tile_capacity(dbl, !(in_row && out_row), wide && !nowide);
     L.T = std::max(1, std::min(std::min(tcap / L.N, I.h.L[u]), 512));
