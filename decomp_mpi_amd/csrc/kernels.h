@@ -122,7 +122,9 @@ int fft_tile_capacity(int dbl, int column, int wide = 0);
 // Rank r's block: count positions per axis at its memory strides; the
 // factor of axis d at local position p is tw[d][first + p] (complex double,
 // (re, im) pairs), multiplied outer axis to inner; zero products skip the
-// slot; conj multiplies by the conjugate.
+// slot; conj multiplies by the conjugate.  A workgroup of `block` threads
+// takes one chunk of one row (the last axis), or block / count whole rows
+// when the rows are shorter; max_blocks: the most workgroups any rank needs.
 struct ScaleLaunch {
   void *data;                  // slot 0 of the buffer (pass precision)
   i64 slot;                    // elements per rank slot
@@ -132,6 +134,8 @@ struct ScaleLaunch {
   const i64 *strides;          // [nranks][ndim]
   const double *tw[DM_MAX_DIMS];
   long long max_elems;         // largest per-rank block
+  long long max_blocks;        // most row chunks of any rank
+  int block;                   // threads per workgroup (a multiple of 64, at most 256)
   int dbl, conj;
 };
 int launch_scale(const ScaleLaunch &p, void *stream);

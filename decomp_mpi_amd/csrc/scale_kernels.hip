@@ -15,74 +15,100 @@
 namespace dm {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kPerThread = 1;  // 2 and 4 measured slower (2.61 and 2.84 against 2.10 ms per 512^3 step)
+constexpr int kThreads = 256;  // most threads per workgroup
 
-// IDX: the integer type of a rank's element index (32-bit unless a rank
-// holds 2^32 elements or more: a 64-bit division costs several times more).
+// A workgroup takes one chunk of one row (the last axis), or, for rows
+// shorter than the workgroup, several whole rows: the threads of a row share
+// its outer coordinates.  The product order is the reference's, outer axes
+// first, so results do not depend on the mapping.
+// IDX: the integer type of a rank's element and chunk indices (32-bit
+// unless a rank holds 2^32 elements or more).
 template <typename R, bool CONJ, typename IDX>
 __global__ __launch_bounds__(kThreads) void scale_kernel(ScaleLaunch p) {
   const int r = blockIdx.y;
   const int nd = p.ndim;
   const int *rt = p.rank_tab + (size_t)r * nd * 2;
   const i64 *st = p.strides + (size_t)r * nd;
-  IDX total = 1;
-  for (int d = 0; d < nd; d++) total *= (IDX)rt[2 * d + 1];
-  typedef R v2 __attribute__((ext_vector_type(2)));
-  v2 *data = reinterpret_cast<v2 *>(p.data) + (i64)r * p.slot;
-  const IDX base = (IDX)blockIdx.x * kThreads * kPerThread + threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < kPerThread; k++) {
-    IDX e = base + (IDX)k * kThreads;
-    if (e >= total) break;
-    // local coordinates, last axis fastest
-    i64 off = 0;
-    int loc[DM_MAX_DIMS];
-    for (int d = nd - 1; d >= 0; d--) {
-      const IDX c = (IDX)rt[2 * d + 1];
-      const IDX q = e / c;
-      loc[d] = (int)(e - q * c);
-      e = q;
-      off += (i64)loc[d] * st[d];
-    }
-    double fr = 1.0, fi = 0.0;
-    for (int d = 0; d < nd; d++) {
-      const double *t = p.tw[d] + 2 * (size_t)(rt[2 * d] + loc[d]);
-      const double tr = t[0], ti = t[1];
-      const double nr = fr * tr - fi * ti;
-      fi = fr * ti + fi * tr;
-      fr = nr;
-    }
-    if (fr == 0.0 && fi == 0.0) continue;
-    if (CONJ) fi = -fi;
-    // each element is read once and written once: non-temporal accesses
-    const v2 x = __builtin_nontemporal_load(data + off);
-    const double xr = (double)x.x, xi = (double)x.y;
-    __builtin_nontemporal_store(v2{(R)(fr * xr - fi * xi), (R)(fr * xi + fi * xr)}, data + off);
+  const int inner = rt[2 * (nd - 1) + 1];
+  if (inner <= 0) return;  // an empty block
+  IDX rows = 1;
+  for (int d = 0; d < nd - 1; d++) rows *= (IDX)rt[2 * d + 1];
+  const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
+  const IDX bx = (IDX)blockIdx.x;
+  IDX row;
+  int col;
+  if (inner >= bd) {  // chunks of one row
+    const IDX cpr = ((IDX)inner + (IDX)bd - 1) / (IDX)bd;
+    if (bx >= rows * cpr) return;
+    row = bx / cpr;
+    col = (int)(bx - row * cpr) * bd + tid;
+    if (col >= inner) return;
+  } else {  // bd / inner whole rows
+    const int rpb = bd / inner, sub = tid / inner;
+    col = tid - sub * inner;
+    row = bx * (IDX)rpb + (IDX)sub;
+    if (sub >= rpb || row >= rows) return;
   }
+  // the row's outer coordinates
+  int loc[DM_MAX_DIMS];
+  i64 off = (i64)col * st[nd - 1];
+  for (int d = nd - 2; d >= 0; d--) {
+    const IDX c = (IDX)rt[2 * d + 1];
+    const IDX q = row / c;
+    loc[d] = (int)(row - q * c);
+    row = q;
+    off += (i64)loc[d] * st[d];
+  }
+  // the element's load goes out before the factors' loads return (every
+  // slot of the box is inside the rank's slot; a zero product only skips
+  // the store), so its latency overlaps theirs
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  v2 *data = reinterpret_cast<v2 *>(p.data) + (i64)r * p.slot + off;
+  const v2 x = __builtin_nontemporal_load(data);
+  double fr = 1.0, fi = 0.0;
+  for (int d = 0; d < nd - 1; d++) {
+    const double *t = p.tw[d] + 2 * (size_t)(rt[2 * d] + loc[d]);
+    const double tr = t[0], ti = t[1];
+    const double nr = fr * tr - fi * ti;
+    fi = fr * ti + fi * tr;
+    fr = nr;
+  }
+  {
+    const double *t = p.tw[nd - 1] + 2 * (size_t)(rt[2 * (nd - 1)] + col);
+    const double tr = t[0], ti = t[1];
+    const double nr = fr * tr - fi * ti;
+    fi = fr * ti + fi * tr;
+    fr = nr;
+  }
+  if (fr == 0.0 && fi == 0.0) return;
+  if (CONJ) fi = -fi;
+  // each element is read once and written once: non-temporal accesses
+  const double xr = (double)x.x, xi = (double)x.y;
+  __builtin_nontemporal_store(v2{(R)(fr * xr - fi * xi), (R)(fr * xi + fi * xr)}, data);
 }
 
 }  // namespace
 
 int launch_scale(const ScaleLaunch &p, void *stream) {
-  if (p.nranks <= 0 || p.max_elems <= 0) return 0;
-  const long long per_block = (long long)kThreads * kPerThread;
-  dim3 grid((unsigned)((p.max_elems + per_block - 1) / per_block), (unsigned)p.nranks);
+  if (p.nranks <= 0 || p.max_elems <= 0 || p.max_blocks <= 0) return 0;
+  if (p.block < 64 || p.block > kThreads || p.block % 64 != 0 || p.max_blocks >= (1LL << 31))
+    return (int)hipErrorInvalidValue;
+  const dim3 grid((unsigned)p.max_blocks, (unsigned)p.nranks), block((unsigned)p.block);
   hipStream_t s = (hipStream_t)stream;
-  if (p.max_elems + per_block >= (1LL << 32)) {
+  if (p.max_elems + p.block >= (1LL << 32)) {
     if (p.dbl) {
-      if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true, unsigned long long>), grid, dim3(kThreads), 0, s, p);
-      else hipLaunchKernelGGL((scale_kernel<double, false, unsigned long long>), grid, dim3(kThreads), 0, s, p);
+      if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true, unsigned long long>), grid, block, 0, s, p);
+      else hipLaunchKernelGGL((scale_kernel<double, false, unsigned long long>), grid, block, 0, s, p);
     } else {
-      if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true, unsigned long long>), grid, dim3(kThreads), 0, s, p);
-      else hipLaunchKernelGGL((scale_kernel<float, false, unsigned long long>), grid, dim3(kThreads), 0, s, p);
+      if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true, unsigned long long>), grid, block, 0, s, p);
+      else hipLaunchKernelGGL((scale_kernel<float, false, unsigned long long>), grid, block, 0, s, p);
     }
   } else if (p.dbl) {
-    if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true, unsigned>), grid, dim3(kThreads), 0, s, p);
-    else hipLaunchKernelGGL((scale_kernel<double, false, unsigned>), grid, dim3(kThreads), 0, s, p);
+    if (p.conj) hipLaunchKernelGGL((scale_kernel<double, true, unsigned>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((scale_kernel<double, false, unsigned>), grid, block, 0, s, p);
   } else {
-    if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true, unsigned>), grid, dim3(kThreads), 0, s, p);
-    else hipLaunchKernelGGL((scale_kernel<float, false, unsigned>), grid, dim3(kThreads), 0, s, p);
+    if (p.conj) hipLaunchKernelGGL((scale_kernel<float, true, unsigned>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((scale_kernel<float, false, unsigned>), grid, block, 0, s, p);
   }
   return (int)hipGetLastError();
 }

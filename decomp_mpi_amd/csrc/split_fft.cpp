@@ -594,6 +594,19 @@ static bool compile_phased(SplitPlan plan) {
       }
       max_elems = std::max(max_elems, n);
     }
+    // 256-thread workgroups: one chunk of one row (the last axis) each, or
+    // several whole rows when the rows are shorter (the kernel's mapping)
+    const int block = 256;
+    long long max_blocks = 0;
+    for (int r = 0; r < nr; r++) {
+      long long rows = 1;
+      for (int d = 0; d < nd - 1; d++) rows *= rt[((size_t)r * nd + d) * 2 + 1];
+      const long long inner = rt[((size_t)r * nd + nd - 1) * 2 + 1];
+      if (rows <= 0 || inner <= 0) continue;  // a rank with an empty block
+      const long long need = inner >= block ? rows * ((inner + block - 1) / block)
+                                            : (rows + block / inner - 1) / (block / inner);
+      max_blocks = std::max(max_blocks, need);
+    }
     dm::TableBuilder tb;
     std::vector<size_t> o_tab(nd);
     for (int d = 0; d < nd; d++) o_tab[d] = tb.put(tabs[d].data(), tabs[d].size() * sizeof(double));
@@ -616,6 +629,8 @@ static bool compile_phased(SplitPlan plan) {
     L.strides = (const i64 *)(b + o_st);
     for (int d = 0; d < nd; d++) L.tw[d] = (const double *)(b + o_tab[d]);
     L.max_elems = max_elems;
+    L.max_blocks = max_blocks;
+    L.block = block;
     L.dbl = x->dbl;
   }
   ph.compiled = true;
