@@ -252,7 +252,8 @@ def test_split_fft_axis_longer_than_a_tile(C, P, shape, grid, B, dbl):
 
 @pytest.mark.parametrize("dbl", [True, False])
 @pytest.mark.parametrize("P,shape,grid", [(4, (16, 16, 16), (1, 2, 2)), (8, (32, 16, 32), (2, 2, 2)),
-                                          (32, (64, 64, 64), (1, 4, 8))])
+                                          (32, (64, 64, 64), (1, 4, 8)),
+                                          (4, (34, 38, 16), (1, 2, 2))])   # primes 17, 19
 def test_autotuned_plan_vs_oracle_and_dft(C, P, shape, grid, dbl):
     """Whatever autotuneParallelPlan[f] returns (split_fft.c:78-247 picks by
     timing; this library by a traffic model plus timing), its forward
