@@ -31,7 +31,6 @@ constexpr int BNTH = 512;
 // 2.7 times as long per byte as a line pass: no memory work overlaps its
 // stages.)
 constexpr int PCAP_F = 8192;
-constexpr int PNTH = 512;
 // Twiddles (16 bytes per root) go to LDS after the tile when every schedule
 // of the launch has lines of at most BTW_LDS_N points; otherwise the launch
 // reads them from device memory.
@@ -341,7 +340,7 @@ __global__ __launch_bounds__(NTH, (PLANE && sizeof(R) == 4 && CAP / NTH == 8) ? 
 
 template <typename R, bool INV, int CAP, bool PRIMES, bool TWL, bool PLANE = false, bool ANY = false>
 hipError_t launch_typed(const BatchLaunch &p, hipStream_t s) {
-  constexpr int NTH = CAP > BCAP_F ? PNTH : BNTH;
+  constexpr int NTH = BNTH;  // every form, the large planes at 16 points per thread
   auto k = batch_pass_kernel<R, INV, CAP, NTH, PRIMES, TWL, PLANE, ANY>;
   // once per kernel (a thread-safe static)
   static const hipError_t attr =
