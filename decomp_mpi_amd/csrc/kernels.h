@@ -133,6 +133,8 @@ struct ScaleLaunch {
   const int *rank_tab;         // [nranks][ndim][2]: first table row, count
   const i64 *strides;          // [nranks][ndim]
   const double *tw[DM_MAX_DIMS];
+  const double *rows;          // per rank, per row: the outer axes' factors' product
+  const long long *row_off;    // [nranks]: rank r's first entry in rows
   long long max_elems;         // largest per-rank block
   long long max_blocks;        // most row chunks of any rank
   int block;                   // threads per workgroup (a multiple of 64, at most 256)

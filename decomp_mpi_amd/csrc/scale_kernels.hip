@@ -18,9 +18,11 @@ namespace {
 constexpr int kThreads = 256;  // most threads per workgroup
 
 // A workgroup takes one chunk of one row (the last axis), or, for rows
-// shorter than the workgroup, several whole rows: the threads of a row share
-// its outer coordinates.  The product order is the reference's, outer axes
-// first, so results do not depend on the mapping.
+// shorter than the workgroup, several whole rows.  The product of a row's
+// outer factors comes from the plan's row table (formed on the host, outer
+// axis first, as the reference's recursion forms it), so an element takes
+// two table loads -- its row's and its last-axis factor -- and two complex
+// products.
 // IDX: the integer type of a rank's element and chunk indices (32-bit
 // unless a rank holds 2^32 elements or more).
 template <typename R, bool CONJ, typename IDX>
@@ -50,14 +52,13 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(ScaleLaunch p) {
     if (sub >= rpb || row >= rows) return;
   }
   // the row's outer coordinates
-  int loc[DM_MAX_DIMS];
+  const IDX row0 = row;
   i64 off = (i64)col * st[nd - 1];
   for (int d = nd - 2; d >= 0; d--) {
     const IDX c = (IDX)rt[2 * d + 1];
     const IDX q = row / c;
-    loc[d] = (int)(row - q * c);
+    off += (i64)(row - q * c) * st[d];
     row = q;
-    off += (i64)loc[d] * st[d];
   }
   // the element's load goes out before the factors' loads return (every
   // slot of the box is inside the rank's slot; a zero product only skips
@@ -65,14 +66,9 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(ScaleLaunch p) {
   typedef R v2 __attribute__((ext_vector_type(2)));
   v2 *data = reinterpret_cast<v2 *>(p.data) + (i64)r * p.slot + off;
   const v2 x = __builtin_nontemporal_load(data);
-  double fr = 1.0, fi = 0.0;
-  for (int d = 0; d < nd - 1; d++) {
-    const double *t = p.tw[d] + 2 * (size_t)(rt[2 * d] + loc[d]);
-    const double tr = t[0], ti = t[1];
-    const double nr = fr * tr - fi * ti;
-    fi = fr * ti + fi * tr;
-    fr = nr;
-  }
+  // the outer axes' product from the plan's row table, then the last axis
+  const double *rp = p.rows + 2 * ((size_t)p.row_off[r] + (size_t)row0);
+  double fr = rp[0], fi = rp[1];
   {
     const double *t = p.tw[nd - 1] + 2 * (size_t)(rt[2 * (nd - 1)] + col);
     const double tr = t[0], ti = t[1];

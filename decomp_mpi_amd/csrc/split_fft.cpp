@@ -607,9 +607,41 @@ static bool compile_phased(SplitPlan plan) {
                                             : (rows + block / inner - 1) / (block / inner);
       max_blocks = std::max(max_blocks, need);
     }
+    // per rank, the product of the outer axes' factors for every row (the
+    // last axis's positions share it), formed outer axis first as the
+    // reference's recursion forms it
+    std::vector<double> rows_tab;
+    std::vector<long long> row_off(nr, 0);
+    for (int r = 0; r < nr; r++) {
+      long long rows = 1;
+      for (int d = 0; d < nd - 1; d++) rows *= rt[((size_t)r * nd + d) * 2 + 1];
+      if (rt[((size_t)r * nd + nd - 1) * 2 + 1] <= 0) rows = 0;
+      row_off[r] = (long long)(rows_tab.size() / 2);
+      for (long long row = 0; row < rows; row++) {
+        long long rest = row;
+        int loc[DM_MAX_DIMS];
+        for (int d = nd - 2; d >= 0; d--) {
+          const int c = rt[((size_t)r * nd + d) * 2 + 1];
+          loc[d] = (int)(rest % c);
+          rest /= c;
+        }
+        double fr = 1.0, fi = 0.0;
+        for (int d = 0; d < nd - 1; d++) {
+          const double *t = &tabs[d][2 * (size_t)(rt[((size_t)r * nd + d) * 2] + loc[d])];
+          const double nr_ = fr * t[0] - fi * t[1];
+          fi = fr * t[1] + fi * t[0];
+          fr = nr_;
+        }
+        rows_tab.push_back(fr);
+        rows_tab.push_back(fi);
+      }
+    }
+    if (rows_tab.empty()) rows_tab.assign(2, 0.0);
     dm::TableBuilder tb;
     std::vector<size_t> o_tab(nd);
     for (int d = 0; d < nd; d++) o_tab[d] = tb.put(tabs[d].data(), tabs[d].size() * sizeof(double));
+    const size_t o_rows = tb.put(rows_tab.data(), rows_tab.size() * sizeof(double));
+    const size_t o_roff = tb.put(row_off.data(), row_off.size() * sizeof(long long));
     const size_t o_rt = tb.put(rt.data(), rt.size() * sizeof(int));
     const size_t o_st = tb.put(hA.strides.data(), hA.strides.size() * sizeof(i64));
     ph.tw_mem = dm::device_alloc(tb.host.size());
@@ -628,6 +660,8 @@ static bool compile_phased(SplitPlan plan) {
     L.rank_tab = (const int *)(b + o_rt);
     L.strides = (const i64 *)(b + o_st);
     for (int d = 0; d < nd; d++) L.tw[d] = (const double *)(b + o_tab[d]);
+    L.rows = (const double *)(b + o_rows);
+    L.row_off = (const long long *)(b + o_roff);
     L.max_elems = max_elems;
     L.max_blocks = max_blocks;
     L.block = block;
