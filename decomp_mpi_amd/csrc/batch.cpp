@@ -291,6 +291,16 @@ bool build_batch_host(int count, int ndim, const int *shapes, int dbl, int tile_
       }
     }
   }
+  // Every other launch walks its tiles in reverse transform order, so that it
+  // starts on the transforms the launch before it wrote last, which are
+  // still in the memory-side cache (tiles are independent: the order changes
+  // no result).  Launch order: planes[0], planes[1], tiles[0], tiles[1], ...
+  std::vector<std::vector<BatchTile> *> seq;
+  for (auto &pl : h.planes)
+    if (!pl.empty()) seq.push_back(&pl);
+  for (auto &tl : h.tiles)
+    if (!tl.empty()) seq.push_back(&tl);
+  for (size_t k = 1; k < seq.size(); k += 2) std::reverse(seq[k]->begin(), seq[k]->end());
   return true;
 }
 
