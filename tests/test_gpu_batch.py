@@ -193,6 +193,31 @@ def test_batch_planes_match_numpy(C, case, dbl, direction):
     assert all(np.array_equal(g, e) for g, e in zip(replay, eager))
 
 
+@pytest.mark.parametrize("dbl", [1, 0])
+@pytest.mark.parametrize("tile", [0, 1, 2])
+def test_batch_schedules_and_grids(C, tile, dbl):
+    """Every schedule (full, half and plane tiles) in both grid modes (one
+    workgroup per tile, persistent workgroups looping over the tile list)
+    against numpy's FP64 DFT. Every other launch walks its tiles from the
+    last transform to the first; the two grid modes run the same tiles
+    through the same kernels, so their results are equal bit for bit."""
+    shapes = SHAPES3 + [(17, 34, 16), (64, 17, 48)]
+    b = Batch(C, shapes, dbl, -1, seed=21)
+    out = []
+    for grid in (0, 1):  # DM_BATCH_GRID_TILES, DM_BATCH_GRID_PERSISTENT
+        assert C.dm_batch_set_schedule(b.plan, tile, 0) == 0
+        assert C.dm_batch_set_grid(b.plan, grid) == 0
+        assert C.dm_batch_grid(b.plan) == grid
+        print(f"schedules: tile {tile} grid {grid} precision {dbl}", flush=True)
+        b.upload()
+        assert C.dm_batch_run(b.plan, None) == 0
+        out.append(b.download())
+    errs = [rel(g, w) for g, w in zip(out[0], b.want())]
+    b.free()
+    assert max(errs) < TOL[dbl], errs
+    assert all(np.array_equal(x, y) for x, y in zip(out[0], out[1]))
+
+
 def test_batch_autotune_keeps_results(C):
     shapes = [(64, 96, 128), (32, 48, 64), (96, 32, 128)]
     b = Batch(C, shapes, 0, -1, seed=11)
