@@ -558,7 +558,7 @@ def main():
                     help="run one section alone and print its figures (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the virtual-rank grid stands in for "
+        print(json.dumps({"error": "bench.py runs on one GPU; the virtual-rank grid stands in for "
                                    "the processor grid (multi-GPU is a design, not a run)"}))
         sys.exit(2)
     import decomp_mpi_amd as dm

@@ -156,7 +156,7 @@ bool plan_pass(dm_comm_t comm, const Side &I, const Side &U, int t, int dbl, boo
   L.N = I.h.L[t];
   int cap = fft_tile_capacity(dbl, 1);
   if (L.N > cap) {
-    raise_error(comm, "FFT: axis %d of length %d exceeds the %d-point tile of this build", t, L.N, cap);
+    raise_error(comm, "FFT: axis %d of length %d exceeds the %d-point pass tile", t, L.N, cap);
     return false;
   }
   std::vector<int> rad;
